@@ -1,0 +1,798 @@
+"""Device runtime of the `hip` target: groups, pathways, monitors, network.
+
+The objects mirror the Brian objects of the clock-driven step (NeuronGroup,
+Synapses with an on_pre pathway, SpikeGeneratorGroup, SpikeMonitor,
+StateMonitor, Network.run) and are driven by Brian's optimised statements:
+either captured specs (``brian2_amd/specs``) or, when brian2 is importable,
+the statements of the user's own objects (``brian2_amd.brian_device``).
+
+State lives in device memory as torch tensors (one per variable, structure of
+arrays); kernels are generated per group/pathway, compiled with hiprtc and
+replayed per step from a hipGraph-captured plan.
+"""
+import ctypes
+import hashlib
+import json
+import os
+
+import numpy as np
+
+from . import _lib
+from .codegen import ctype_of
+from .kernels import (
+    T_TILE,
+    CodeSpec,
+    GatherKernel,
+    NeuronKernel,
+    identifiers,
+    scalar_values,
+)
+
+L = _lib.lib
+SPEC_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "specs")
+
+_NP = {"double": np.float64, "float": np.float32, "int": np.int32, "long long": np.int64, "bool": np.uint8}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+_TDT = None
+
+
+def torch_dtype(ct):
+    torch = _torch()
+    return {"double": torch.float64, "float": torch.float32, "int": torch.int32, "long long": torch.int64,
+            "bool": torch.uint8, "unsigned long long": torch.int64, "unsigned int": torch.int32}[ct]
+
+
+# ---------------------------------------------------------------------------
+class Device:
+    """One HIP context per process (device 0 unless given)."""
+
+    _instance = None
+
+    def __init__(self, device=0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("the hip target needs a GPU (torch.cuda.is_available() is False)")
+        torch.cuda.set_device(device)
+        self.device = device
+        self.ctx = ctypes.c_void_p()
+        _lib.check(L.hb_ctx_create(device, ctypes.byref(self.ctx)), "hb_ctx_create")
+        self._modules = {}
+
+    @classmethod
+    def get(cls):
+        if cls._instance is None:
+            cls._instance = Device()
+        return cls._instance
+
+    @property
+    def stream(self):
+        return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+    def function(self, source, name):
+        key = hashlib.sha1(source.encode()).hexdigest()
+        mod = self._modules.get(key)
+        if mod is None:
+            mod = ctypes.c_void_p()
+            log = ctypes.create_string_buffer(1 << 16)
+            rc = L.hb_module_load(self.ctx, source.encode(), b"", ctypes.byref(mod), log, len(log))
+            if rc != 0:
+                raise _lib.HipBrianError("kernel compilation failed:\n" + _lib.last_error() + "\n--- source ---\n"
+                                         + source)
+            self._modules[key] = mod
+        fn = ctypes.c_void_p()
+        _lib.check(L.hb_module_function(mod, name.encode(), ctypes.byref(fn)), f"function {name}")
+        return fn
+
+    def sync(self):
+        _torch().cuda.synchronize()
+
+
+def dev(arr, ct=None):
+    torch = _torch()
+    a = np.ascontiguousarray(arr)
+    if ct is not None:
+        a = a.astype(_NP[ct], copy=False)
+    return torch.from_numpy(a.copy()).to("cuda")
+
+
+def zeros(n, ct):
+    return _torch().zeros(max(int(n), 1), dtype=torch_dtype(ct), device="cuda")
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+# ---------------------------------------------------------------------------
+class Model:
+    """Brian's statements for one model: group code objects and pathway code objects."""
+
+    def __init__(self, d):
+        self.name = d.get("model", "model")
+        self.group = {k: CodeSpec(v) for k, v in d["group"].items()}
+        self.pathways = {k: CodeSpec(v) for k, v in d.get("pathways", {}).items()}
+
+    @classmethod
+    def load(cls, name):
+        path = name if os.path.exists(name) else os.path.join(SPEC_DIR, f"{name}.json")
+        with open(path) as f:
+            return cls(json.load(f))
+
+
+# ---------------------------------------------------------------------------
+_names = {}
+
+
+def _unique(base):
+    k = _names.get(base, 0)
+    _names[base] = k + 1
+    return base if k == 0 else f"{base}_{k}"
+
+
+class _GroupBase:
+    def __getitem__(self, sl):
+        if not isinstance(sl, slice) or sl.step not in (None, 1):
+            raise TypeError("subgroups need a contiguous slice")
+        a, b, _ = sl.indices(self.N)
+        return Subgroup(self, a, b)
+
+
+class Subgroup:
+    """Contiguous slice of a group (brian2/groups/subgroup.py:10-60)."""
+
+    def __init__(self, group, start, stop):
+        self.group, self.start, self.stop = group, start, stop
+        self.N = stop - start
+
+
+class NeuronGroup(_GroupBase):
+    """State, stateupdate, threshold and reset of one group.
+
+    ``model`` holds the group's stateupdate/threshold/reset code objects.
+    ``namespace`` overrides the values of Brian Constants (e.g. a parameter sweep).
+    ``float_t`` is "double" (fp64 verification mode) or "float" (fp32 mode).
+    """
+
+    def __init__(self, N, model, float_t="double", name=None, namespace=None):
+        if isinstance(model, str):
+            model = Model.load(model)
+        self.N = int(N)
+        self.name = name or _unique("neurongroup")
+        self.float_t = float_t
+        self.su = model.group.get("stateupdate")
+        self.th = model.group.get("threshold")
+        self.rs = model.group.get("reset")
+        self.namespace = dict(namespace or {})
+        self.vars = {}
+        owner = None
+        for spec in (self.su, self.th, self.rs):
+            if spec is None:
+                continue
+            for n, v in spec.variables.items():
+                if v["kind"] in ("ArrayVariable", "DynamicArrayVariable") and not v.get("scalar") \
+                        and spec.indices.get(n, "_idx") == "_idx" and n not in ("i", "N") \
+                        and v.get("owner") not in (None, "defaultclock"):
+                    owner = owner or v["owner"]
+                    if v["owner"] == owner:
+                        self.vars[n] = ctype_of(v["dtype"], float_t, n)
+        self.vars = dict(sorted(self.vars.items()))
+        self.uses_refractory = "lastspike" in self.vars and "not_refractory" in self.vars
+        self._host = {}
+        for n, ct in self.vars.items():
+            if n == "lastspike":
+                self._host[n] = np.full(self.N, -1e4)
+            elif n == "not_refractory":
+                self._host[n] = np.ones(self.N, np.uint8)
+            else:
+                self._host[n] = np.zeros(self.N, _NP[ct])
+        self._dev = None
+        self.net = None
+
+    # state access (VariableView get/set, brian2/core/variables.py:946)
+    def get(self, name):
+        if self._dev is not None:
+            return self._dev[name].cpu().numpy().copy()
+        return self._host[name].copy()
+
+    def set(self, name, values):
+        if name not in self.vars:
+            raise KeyError(f"{self.name} has no state variable {name!r}")
+        a = np.broadcast_to(np.asarray(values), (self.N,)).astype(_NP[self.vars[name]])
+        if self._dev is not None:
+            self._dev[name].copy_(dev(a))
+        else:
+            self._host[name] = a.copy()
+
+    def __getattr__(self, k):
+        if k.startswith("_") or k not in self.__dict__.get("vars", {}):
+            raise AttributeError(k)
+        return self.get(k)
+
+    def __setattr__(self, k, v):
+        if k in self.__dict__.get("vars", {}):
+            self.set(k, v)
+        else:
+            object.__setattr__(self, k, v)
+
+    def state_tensor(self, name):
+        """The device tensor of a state variable (zero-copy handoff)."""
+        return self._dev[name]
+
+
+class SpikeGeneratorGroup(_GroupBase):
+    """Replays (index, time) spikes (brian2/input/spikegeneratorgroup.py:21)."""
+
+    def __init__(self, N, indices, times=None, timebins=None, dt=1e-4, period_bins=0, name=None):
+        self.N = int(N)
+        self.name = name or _unique("spikegeneratorgroup")
+        idx = np.asarray(indices, np.int32)
+        if timebins is None:
+            # spikegeneratorgroup.py:241-245: shift by 1e-3 dt, then truncate
+            times = np.asarray(times, np.float64)
+            timebins = np.asarray((times + 1e-3 * dt) / dt, dtype=np.int32)
+        tb = np.asarray(timebins, np.int32)
+        order = np.lexsort((idx, tb))
+        self.indices, self.timebins = idx[order], tb[order]
+        self.period_bins = int(period_bins)
+        self.vars = {}
+        self._dev = None
+        self.net = None
+        self.uses_refractory = False
+
+
+class Synapses:
+    """One synaptic object with an on_pre pathway.
+
+    ``model`` is the pathway's synapses code object (CodeSpec).  Connectivity is
+    given explicitly with connect(i, j) (brian2/synapses/synapses.py:1704-1710,
+    synapse index = array order), per-synapse variables with set(), and delays
+    in seconds with delay= (scalar or per synapse).  ``storage`` may store a
+    float64 synaptic variable as float32 on the device when every value is
+    exactly representable (checked)."""
+
+    def __init__(self, source, target, model, name=None, storage=None, order=0):
+        self.name = name or _unique("synapses")
+        self.pathway_name = f"{self.name}_pre"
+        self.order = order
+        if isinstance(source, Subgroup):
+            self.source, self.src_range = source.group, (source.start, source.stop)
+        else:
+            self.source, self.src_range = source, (0, source.N)
+        if isinstance(target, Subgroup):
+            raise NotImplementedError("subgroup targets are not supported by the hip target yet")
+        self.target = target
+        self.spec = model
+        self.storage = dict(storage or {})
+        self._analyse()
+        self.pre = None
+        self.post = None
+        self.delay = 0.0
+        self.syn = {}
+        self.csr = None
+        self.net = None
+
+    def _analyse(self):
+        vs = self.spec.vector
+        if not vs:
+            raise NotImplementedError("empty on_pre code")
+        last = vs[-1]
+        idx = self.spec.indices
+        if idx.get(last["var"]) != "_postsynaptic_idx" or last["op"] not in ("+=", "-=", "*=", "/=", "="):
+            raise NotImplementedError("on_pre code must end with an update of a postsynaptic variable")
+        for st in vs[:-1]:
+            if st["op"] != ":=":
+                raise NotImplementedError("on_pre code may only define temporaries before the target update")
+        used = identifiers(vs) - {last["var"]}
+        self.syn_vars = []
+        for n in sorted(used):
+            v = self.spec.variables.get(n)
+            if v is None or v["kind"] in ("Constant", "Function", "AuxiliaryVariable"):
+                continue
+            ix = idx.get(n, "_idx")
+            if ix == "_postsynaptic_idx" or ix == "_presynaptic_idx":
+                raise NotImplementedError(f"on_pre expression reads {n!r} of a neuron group")
+            if n == "_postsynaptic_idx":
+                continue
+            self.syn_vars.append(n)
+        self.target_var = last["var"]
+        self.op = last["op"]
+
+    def connect(self, i, j):
+        i = np.asarray(i, np.int64)
+        j = np.asarray(j, np.int64)
+        n_src = self.src_range[1] - self.src_range[0]
+        if len(i) and (i.min() < 0 or i.max() >= n_src or j.min() < 0 or j.max() >= self.target.N):
+            raise IndexError("synapse indices out of range")
+        self.pre = (i + self.src_range[0]).astype(np.int32)
+        self.post = j.astype(np.int32)
+
+    def set(self, name, values):
+        if self.pre is None:
+            raise RuntimeError("connect() first")
+        a = np.broadcast_to(np.asarray(values, np.float64), (len(self.pre),)).copy()
+        self.syn[name] = a
+
+    def set_delay(self, delay):
+        self.delay = delay
+
+    def connect_device_csr(self, seg_ptr, post, syn, dmin, n_seg, n_syn):
+        """Connectivity built on the device (see synth.py): CSR by (source, delay)."""
+        self.csr = dict(seg_ptr=seg_ptr, post=post, syn=syn, dmin=int(dmin), n_seg=int(n_seg), n_syn=int(n_syn))
+
+    # CSpikeQueue::prepare counterpart (spikequeue.h:48-105) on the host
+    def _build_csr(self, dt):
+        if self.csr is not None:
+            return
+        S = len(self.pre)
+        d = self.delay
+        if np.ndim(d) == 0:
+            dsteps = np.full(S, int(float(d) / dt + 0.5), np.int64)  # spikequeue.h:90
+        else:
+            dd = np.asarray(d, np.float64)
+            dsteps = (dd / dt + 0.5).astype(np.int64)
+        dmin = int(dsteps.min()) if S else 0
+        dmax = int(dsteps.max()) if S else 0
+        n_seg = dmax - dmin + 1
+        nsrc = self.src_range[1] - self.src_range[0]
+        seg = (self.pre.astype(np.int64) - self.src_range[0]) * n_seg + (dsteps - dmin)
+        order = np.lexsort((np.arange(S), seg))  # per (source, delay): synapse order
+        counts = np.bincount(seg, minlength=nsrc * n_seg)
+        seg_ptr = np.zeros(nsrc * n_seg + 1, np.uint64)
+        np.cumsum(counts, out=seg_ptr[1:])
+        if S >= 2**32 - 1:
+            raise NotImplementedError("more than 2^32 - 1 synapses in one pathway")
+        syn = {}
+        for n in self.syn_vars:
+            if n not in self.syn:
+                raise RuntimeError(f"synaptic variable {n!r} of {self.name} was not set")
+            v = self.syn[n][order]
+            st = self.storage.get(n)
+            if st is not None and np.dtype(st) == np.float32:
+                v32 = v.astype(np.float32)
+                if not np.array_equal(v32.astype(np.float64), v):
+                    raise ValueError(f"{n!r} is not exactly representable in float32")
+                syn[n] = dev(v32)
+            else:
+                syn[n] = dev(v)
+        self.csr = dict(seg_ptr=dev(seg_ptr.astype(np.uint32).view(np.int32)), post=dev(self.post[order]),
+                        syn=syn, dmin=dmin, n_seg=n_seg, n_syn=S, order=order)
+
+    def syn_ctypes(self):
+        out = {}
+        for n in self.syn_vars:
+            t = self.csr["syn"][n]
+            out[n] = "float" if t.dtype == _torch().float32 else "double"
+        return out
+
+
+class SpikeMonitor:
+    """Records (i, t) of spikes and a per-neuron count (brian2/monitors/spikemonitor.py:472)."""
+
+    def __init__(self, source, name=None):
+        if isinstance(source, Subgroup):
+            self.source, self.range = source.group, (source.start, source.stop)
+        else:
+            self.source, self.range = source, (0, source.N)
+        self.name = name or _unique("spikemonitor")
+        self._i = []
+        self._step = []
+        self.count_dev = None
+        self.net = None
+
+    def _collect(self, n_new):
+        pass
+
+    @property
+    def i(self):
+        i, s = self._sorted()
+        return i
+
+    @property
+    def step(self):
+        i, s = self._sorted()
+        return s
+
+    def t(self, dt):
+        return self.step * dt
+
+    def _sorted(self):
+        if self.net is not None:
+            self.net._drain_monitors()
+        i = np.concatenate(self._i) if self._i else np.zeros(0, np.int32)
+        s = np.concatenate(self._step) if self._step else np.zeros(0, np.int64)
+        o = np.lexsort((i, s))  # spikes of one step are ascending (threshold.pyx:18-32)
+        return i[o], s[o]
+
+    @property
+    def count(self):
+        return self.count_dev.cpu().numpy()[: self.range[1] - self.range[0]].copy()
+
+    @property
+    def num_spikes(self):
+        return len(self.i)
+
+
+class StateMonitor:
+    """Records variables at the start of every step (brian2/monitors/statemonitor.py:82)."""
+
+    def __init__(self, source, variables, record, name=None):
+        self.source = source
+        self.variables = list(variables) if not isinstance(variables, str) else [variables]
+        rec = np.asarray(record, np.int32)
+        self.record = rec
+        self.sorted_idx = np.unique(rec)
+        self.name = name or _unique("statemonitor")
+        self._rows = []
+        self.net = None
+
+    def values(self):
+        """Array [n_vars, n_record, n_steps] in the order of `record`."""
+        if not self._rows:
+            return np.zeros((len(self.variables), len(self.record), 0))
+        a = np.concatenate(self._rows, axis=0)  # [steps, vars, sorted_idx]
+        pos = np.searchsorted(self.sorted_idx, self.record)
+        return np.transpose(a[:, :, pos], (1, 2, 0))
+
+
+# ---------------------------------------------------------------------------
+class Network:
+    """Runs the step loop on the device (Network.run, brian2/core/network.py:1047-1292)."""
+
+    def __init__(self, *objs, dt=1e-4, bucket_cap=2048, graph_steps=100):
+        self.objs = []
+        self.dt = float(dt)
+        self.bucket_cap = int(bucket_cap)
+        self.graph_steps = int(graph_steps)
+        self.t_step = 0
+        self._built = False
+        for o in objs:
+            self.add(o)
+
+    def add(self, *objs):
+        for o in objs:
+            if self._built:
+                raise RuntimeError("cannot add objects after the first run")
+            o.net = self
+            self.objs.append(o)
+
+    @property
+    def t(self):
+        return self.t_step * self.dt
+
+    # -- build --------------------------------------------------------------
+    def _build(self):
+        torch = _torch()
+        D = self.device = Device.get()
+        groups = [o for o in self.objs if isinstance(o, (NeuronGroup, SpikeGeneratorGroup))]
+        paths = sorted([o for o in self.objs if isinstance(o, Synapses)], key=lambda s: (s.order, s.pathway_name))
+        smons = [o for o in self.objs if isinstance(o, SpikeMonitor)]
+        stmons = [o for o in self.objs if isinstance(o, StateMonitor)]
+        for s in paths:
+            for g in (s.source, s.target):
+                if g not in groups:
+                    raise RuntimeError(f"{g.name} used by {s.name} is not in the network")
+        self.groups, self.paths, self.smons, self.stmons = groups, paths, smons, stmons
+        self.clock = torch.zeros(1, dtype=torch.int64, device="cuda")
+        self.clock.fill_(self.t_step)
+        self.status = torch.zeros(1, dtype=torch.int32, device="cuda")
+        for s in paths:
+            s._build_csr(self.dt)
+        # per group: device state, spike ring
+        for g in groups:
+            dmax = max([s.csr["dmin"] + s.csr["n_seg"] - 1 for s in paths if s.source is g] + [0])
+            g.ring_slots = dmax + 2
+            g.ring_spk = zeros(g.ring_slots * g.N, "int")
+            g.ring_cnt = zeros(g.ring_slots, "int")
+            if isinstance(g, NeuronGroup):
+                g._dev = {n: dev(g._host[n], ct) for n, ct in g.vars.items()}
+        # incoming pathways -> buckets
+        for g in groups:
+            incoming = [s for s in paths if s.target is g]
+            g.incoming = incoming
+            if not incoming:
+                continue
+            if not isinstance(g, NeuronGroup):
+                raise NotImplementedError("pathways into a spike generator")
+            val_t = "float"
+            for s in incoming:
+                vt = self._value_type(s)
+                if vt == "double":
+                    val_t = "double"
+            g.val_t = val_t
+            g.n_tiles = (g.N + T_TILE - 1) // T_TILE
+            g.bkt_cnt = zeros(g.n_tiles, "int")
+            g.bkt_key = torch.zeros(g.n_tiles * self.bucket_cap, dtype=torch.int64, device="cuda")
+            g.bkt_val = zeros(g.n_tiles * self.bucket_cap, val_t)
+        self._plan = ctypes.c_void_p()
+        _lib.check(L.hb_plan_create(D.ctx, ptr(self.clock), ctypes.byref(self._plan)), "hb_plan_create")
+        self._launches = []
+        for g in groups:
+            if isinstance(g, NeuronGroup):
+                self._build_group(g)
+            else:
+                self._build_spikegen(g)
+        for s in paths:
+            self._build_gather(s)
+        self._built = True
+
+    def _value_type(self, s):
+        from .codegen import Renderer
+
+        types = dict(s.syn_ctypes())
+        for n, v in s.spec.variables.items():
+            if v["kind"] == "Constant":
+                types[n] = "double"
+        for st in s.spec.scalar:
+            types[st["var"]] = ctype_of(st["dtype"], s.target.float_t, st["var"])
+        for st in s.spec.vector[:-1]:
+            types[st["var"]] = ctype_of(st["dtype"], s.target.float_t, st["var"])
+        _, t = Renderer(types).render(s.spec.vector[-1]["expr"])
+        tv = s.target.vars[s.target_var]
+        if tv == "float":
+            return "float"
+        return "float" if t == "float" else "double"
+
+    def _group_scalars(self, g):
+        host = {"dt": self.dt, "N": g.N}
+        host.update(g.namespace)
+        sc = {}
+        for tag, spec in (("su", g.su), ("th", g.th), ("rs", g.rs)):
+            if spec is None:
+                continue
+            vals = scalar_values(spec, host, g.float_t)
+            for n, vc in vals.items():
+                if n in g.namespace:
+                    vc = (g.namespace[n], vc[1])
+                sc[(tag, n)] = vc
+        return sc
+
+    def _reset_mode(self, g):
+        if g.rs is None:
+            return "none"
+        written = set(g.rs.vector_write) | {st["var"] for st in g.rs.vector}
+        read = set(g.rs.vector_read) | identifiers(g.rs.vector)
+        touched = {s.target_var for s in g.incoming} if getattr(g, "incoming", None) else set()
+        return "deferred" if (written | read) & touched else "fused"
+
+    def _build_group(self, g):
+        D = self.device
+        scalars = self._group_scalars(g)
+        fold = []
+        for rank, s in enumerate(g.incoming):
+            fold.append((rank, s.target_var, s.op))
+        mode = self._reset_mode(g)
+        if mode == "deferred":
+            g.spiked = zeros(g.N, "bool")
+        smon = [m for m in self.smons if m.source is g]
+        if len(smon) > 1:
+            raise NotImplementedError("more than one SpikeMonitor per group")
+        stmon = [m for m in self.stmons if m.source is g]
+        if len(stmon) > 1:
+            raise NotImplementedError("more than one StateMonitor per group")
+        g.smon = smon[0] if smon else None
+        g.stmon = stmon[0] if stmon else None
+        for st in (g.stmon,) if g.stmon else ():
+            for v in st.variables:
+                if v not in g.vars:
+                    raise KeyError(f"StateMonitor variable {v!r} is not a state variable of {g.name}")
+        K = NeuronKernel(g.name, g.N, g.vars, g.float_t, g.su, g.th, g.rs, mode, scalars, fold,
+                         self.bucket_cap, getattr(g, "val_t", "double"),
+                         spikemon=g.smon.range if g.smon else None,
+                         statemon=g.stmon.variables if g.stmon else None,
+                         uses_refractory=g.uses_refractory)
+        g.kernel = K
+        g.fn = D.function(K.source, K.kernel_name)
+        g.struct_cls = K.struct.ctypes_struct()
+        if g.smon:
+            m = g.smon
+            m.cap = max(1 << 20, 2 * g.N * self.graph_steps)
+            m.mon_i = zeros(m.cap, "int")
+            m.mon_step = zeros(m.cap, "int")
+            m.mon_n = zeros(1, "long long")
+            m.count_dev = zeros(m.range[1] - m.range[0], "int")
+            m._drained = 0
+        if g.stmon:
+            m = g.stmon
+            m.rec_idx = dev(m.sorted_idx, "int")
+            m.rec_buf = None
+        g.launch_index = len(self._launches)
+        args = self._group_args(g)
+        self._add_launch(g.fn, (g.N + T_TILE - 1) // T_TILE, T_TILE, args, g.struct_cls)
+        if g.incoming or mode == "deferred":
+            F = NeuronKernel(g.name, g.N, g.vars, g.float_t, g.su, g.th, g.rs, mode, scalars, fold,
+                             self.bucket_cap, getattr(g, "val_t", "double"), flush=True,
+                             uses_refractory=g.uses_refractory)
+            g.flush_fn = D.function(F.source, F.kernel_name)
+            g.flush_struct_cls = F.struct.ctypes_struct()
+        else:
+            g.flush_fn = None
+
+    def _group_args(self, g, flush=False):
+        cls = g.flush_struct_cls if flush else g.struct_cls
+        v = {"clock": self.clock.data_ptr(), "step_off": 0, "dt": self.dt, "N": g.N, "ring_slots": g.ring_slots,
+             "ring_spk": g.ring_spk.data_ptr(), "ring_cnt": g.ring_cnt.data_ptr(), "ring_stride": g.N,
+             "status": self.status.data_ptr()}
+        for n in g.vars:
+            v[f"a_{n}"] = g._dev[n].data_ptr()
+        for (tg, n), (val, ct) in g.kernel.scalars.items():
+            v[f"{tg}_{n}"] = val
+        if g.incoming:
+            v.update(bkt_cnt=g.bkt_cnt.data_ptr(), bkt_key=g.bkt_key.data_ptr(), bkt_val=g.bkt_val.data_ptr())
+        if getattr(g, "spiked", None) is not None:
+            v["spiked"] = g.spiked.data_ptr()
+        if not flush and g.smon:
+            m = g.smon
+            v.update(mon_i=m.mon_i.data_ptr(), mon_step=m.mon_step.data_ptr(), mon_n=m.mon_n.data_ptr(),
+                     mon_count=m.count_dev.data_ptr(), mon_cap=m.cap, mon_start=m.range[0], mon_stop=m.range[1])
+        if not flush and g.stmon:
+            m = g.stmon
+            v.update(rec_idx=m.rec_idx.data_ptr(), rec_buf=m.rec_buf.data_ptr() if m.rec_buf is not None else 0,
+                     rec_row0=getattr(m, "row0", 0), rec_rows=getattr(m, "rows", 0), n_rec=len(m.sorted_idx))
+        s = cls()
+        for name, _ in cls._fields_:
+            if name in v:
+                setattr(s, name, v[name])
+        return s
+
+    def _build_spikegen(self, g):
+        fn = ctypes.c_void_p()
+        _lib.check(L.hb_spikegen_function(ctypes.byref(fn)))
+        g.fn = fn
+        g.gen_tb = dev(g.timebins, "int")
+        g.gen_idx = dev(g.indices, "int")
+        g.gen_state = zeros(1, "long long")
+        first = int(np.searchsorted(g.timebins, self.t_step, side="left"))
+        g.gen_state.fill_(first)
+
+        class A(ctypes.Structure):
+            _fields_ = [("timebins", ctypes.c_void_p), ("indices", ctypes.c_void_p), ("n", ctypes.c_int64),
+                        ("state", ctypes.c_void_p), ("ring_spk", ctypes.c_void_p), ("ring_cnt", ctypes.c_void_p),
+                        ("clock", ctypes.c_void_p), ("step_offset", ctypes.c_int64), ("ring_slots", ctypes.c_int32),
+                        ("ring_stride", ctypes.c_int32), ("period_bins", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+        a = A(g.gen_tb.data_ptr(), g.gen_idx.data_ptr(), len(g.timebins), g.gen_state.data_ptr(),
+              g.ring_spk.data_ptr(), g.ring_cnt.data_ptr(), self.clock.data_ptr(), 0, g.ring_slots, g.N,
+              g.period_bins, 0)
+        g.struct_cls = A
+        g.launch_index = len(self._launches)
+        self._add_launch(fn, 1, 256, a, A, step_field="step_offset")
+        g.flush_fn = None
+        g.smon = None
+        g.stmon = None
+
+    def _build_gather(self, s):
+        D = self.device
+        g = s.target
+        rank = g.incoming.index(s)
+        host = {"dt": self.dt}
+        host.update(getattr(s, "namespace", {}) or {})
+        sc = scalar_values(s.spec, host, g.float_t)
+        types_syn = s.syn_ctypes()
+        K = GatherKernel(s.name, rank, s.csr["n_seg"], s.csr["dmin"], s.spec.vector, types_syn, sc, g.val_t,
+                         self.bucket_cap, g.float_t, s.target_var)
+        s.kernel = K
+        s.fn = D.function(K.source, K.kernel_name)
+        cls = K.struct.ctypes_struct()
+        s.struct_cls = cls
+        v = {"clock": self.clock.data_ptr(), "step_off": 0, "ring_spk": s.source.ring_spk.data_ptr(),
+             "ring_cnt": s.source.ring_cnt.data_ptr(), "ring_stride": s.source.N, "ring_slots": s.source.ring_slots,
+             "src0": s.src_range[0], "src1": s.src_range[1], "seg_ptr": s.csr["seg_ptr"].data_ptr(),
+             "post": s.csr["post"].data_ptr(), "bkt_cnt": g.bkt_cnt.data_ptr(), "bkt_key": g.bkt_key.data_ptr(),
+             "bkt_val": g.bkt_val.data_ptr(), "status": self.status.data_ptr()}
+        for n in s.syn_vars:
+            v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
+        for n, (val, ct) in sc.items():
+            v[f"c_{n}"] = val
+        a = cls()
+        for name, _ in cls._fields_:
+            if name in v:
+                setattr(a, name, v[name])
+        nsrc = s.src_range[1] - s.src_range[0]
+        grid = int(min(2048, max(4, (nsrc * s.csr["n_seg"]) // 256)))
+        s.launch_index = len(self._launches)
+        self._add_launch(s.fn, grid, 256, a, cls)
+
+    def _add_launch(self, fn, grid, block, args, cls, step_field="step_off"):
+        off = getattr(cls, step_field).offset
+        buf = bytes(args)
+        _lib.check(L.hb_plan_add(self._plan, fn, grid, block, 0, buf, len(buf), off), "hb_plan_add")
+        self._launches.append((fn, grid, block, cls, off))
+
+    def _set_launch_args(self, index, args):
+        buf = bytes(args)
+        _lib.check(L.hb_plan_set_args(self._plan, index, buf, len(buf)), "hb_plan_set_args")
+
+    # -- run ------------------------------------------------------------------
+    def run(self, n_steps=None, duration=None, graph_steps=None):
+        """Advances the network by n_steps time steps (or duration seconds)."""
+        torch = _torch()
+        if n_steps is None:
+            n_steps = int(round(duration / self.dt))
+        n_steps = int(n_steps)
+        if not self._built:
+            self._build()
+        gs = int(graph_steps or self.graph_steps)
+        stream = self.device.stream
+        # state monitors: one device row per step of this run
+        for g in self.groups:
+            m = getattr(g, "stmon", None)
+            if m is not None:
+                m.rows = n_steps
+                m.row0 = self.t_step
+                m.rec_buf = torch.zeros(max(n_steps, 1) * len(m.variables) * len(m.sorted_idx), dtype=torch.float64,
+                                        device="cuda")
+                self._set_launch_args(g.launch_index, self._group_args(g))
+        done = 0
+        while done < n_steps:
+            chunk = n_steps - done
+            for g in self.groups:
+                m = getattr(g, "smon", None)
+                if m is not None:
+                    chunk = min(chunk, self._ensure_monitor_room(g, m, gs))
+            k = min(chunk, gs * max(1, chunk // gs)) if chunk >= gs else chunk
+            _lib.check(L.hb_plan_run(self._plan, stream, k, gs if k >= gs else 1), "hb_plan_run")
+            done += k
+            self.t_step += k
+            st = int(self.status.cpu().item())  # synchronises
+            if st:
+                raise _lib.HipBrianError(self._status_message(st))
+        # fold the events delivered by the last step into the state (end of run)
+        for g in self.groups:
+            if getattr(g, "flush_fn", None) is not None:
+                a = self._group_args(g, flush=True)
+                off = type(a).step_off.offset
+                a.step_off = 0
+                buf = bytes(a)
+                grid = (g.N + T_TILE - 1) // T_TILE
+                _lib.check(L.hb_launch(g.flush_fn, grid, T_TILE, 0, stream, buf, len(buf)), "flush")
+        for g in self.groups:
+            m = getattr(g, "stmon", None)
+            if m is not None and n_steps > 0:
+                nv, nr = len(m.variables), len(m.sorted_idx)
+                m._rows.append(m.rec_buf.view(n_steps, nv, nr).cpu().numpy().copy())
+        self._drain_monitors()
+        st = int(self.status.cpu().item())
+        if st:
+            raise _lib.HipBrianError(self._status_message(st))
+
+    def _status_message(self, st):
+        msg = []
+        if st & 1:
+            msg.append(f"event bucket overflow (more than bucket_cap={self.bucket_cap} synaptic events for one tile of "
+                       f"{T_TILE} neurons in one step); build the Network with a larger bucket_cap")
+        if st & 2:
+            msg.append("spike monitor buffer overflow")
+        return "; ".join(msg) or f"device status {st}"
+
+    def _ensure_monitor_room(self, g, m, gs):
+        n = int(m.mon_n.cpu().item())
+        free = m.cap - n
+        if free < g.N * gs:
+            torch = _torch()
+            newcap = max(2 * m.cap, n + 2 * g.N * gs)
+            ni = zeros(newcap, "int")
+            ns = zeros(newcap, "int")
+            ni[:n] = m.mon_i[:n]
+            ns[:n] = m.mon_step[:n]
+            m.mon_i, m.mon_step, m.cap = ni, ns, newcap
+            self._set_launch_args(g.launch_index, self._group_args(g))
+            free = m.cap - n
+        return max(1, free // max(g.N, 1))
+
+    def _drain_monitors(self):
+        for g in getattr(self, "groups", []):
+            m = getattr(g, "smon", None)
+            if m is None:
+                continue
+            n = int(m.mon_n.cpu().item())
+            if n > m._drained:
+                m._i.append(m.mon_i[m._drained:n].cpu().numpy().copy())
+                m._step.append(m.mon_step[m._drained:n].cpu().numpy().astype(np.int64))
+                m._drained = n

@@ -1,0 +1,131 @@
+/* hipbrian.h -- C ABI of libhipbrian.so, the MI355X (gfx950) back end of the
+ * `hip` code generation target for Brian2's clock-driven step.
+ *
+ * All functions return 0 on success and a negative code on failure; the
+ * message is then available from hb_last_error() (per thread).  Device
+ * pointers are plain `void*` obtained from hb_malloc or from any other HIP
+ * allocation (e.g. a torch tensor's data_ptr()).  Streams are `void*`
+ * hipStream_t handles; NULL means the default stream.
+ *
+ * Reference counterparts (paths relative to the brian2 source tree):
+ *  - The per-model kernels are Brian's code objects: hb_module_load replaces
+ *    CythonCodeObject.compile_block (codegen/runtime/cython_rt/cython_rt.py:177-193)
+ *    and hb_launch replaces CythonCodeObject.run_block (cython_rt.py:195-205).
+ *  - hb_plan_run replaces the Network.run hot loop (core/network.py:1194-1250)
+ *    together with Clock.advance (core/clocks.py:315-325): the ordered per-step
+ *    code objects are replayed n_steps times on the device.
+ *  - hb_csr_* replace CSpikeQueue::prepare (synapses/spikequeue.h:48-105): per-source
+ *    synapse lists in synapse order, with the integer delay (int)(d/dt + 0.5)
+ *    (spikequeue.h:90) made a segment of each list.
+ *  - hb_spikegen_step replaces the spikegenerator template
+ *    (codegen/runtime/cython_rt/templates/spikegenerator.pyx:6-26).
+ */
+#ifndef HIPBRIAN_H
+#define HIPBRIAN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hb_ctx hb_ctx;
+typedef struct hb_module hb_module;
+typedef struct hb_plan hb_plan;
+
+/* ---- errors / version --------------------------------------------------- */
+int hb_version(void);
+/* Copies the calling thread's last error message; returns its full length. */
+int hb_last_error(char* buf, size_t n);
+
+/* ---- context ------------------------------------------------------------ */
+/* One context per device: owns a default-priority stream used for graph
+ * replay, and the modules/plans created from it.  (Device.activate,
+ * devices/device.py:86-467, RuntimeDevice 470-615) */
+int hb_ctx_create(int device, hb_ctx** out);
+int hb_ctx_destroy(hb_ctx* ctx);
+int hb_device_count(int* n);
+
+/* ---- memory (RuntimeDevice.add_array / get_value / set_value,
+ *      devices/device.py:500-555) ---------------------------------------- */
+int hb_malloc(hb_ctx* ctx, size_t nbytes, void** dptr);
+int hb_free(hb_ctx* ctx, void* dptr);
+/* kind: 1 host->device, 2 device->host, 3 device->device (ordered on stream) */
+int hb_memcpy(void* dst, const void* src, size_t nbytes, int kind, void* stream);
+/* Fills nbytes (multiple of 4) with a 32-bit pattern using a kernel. */
+int hb_fill_u32(void* dptr, uint32_t value, size_t nbytes, void* stream);
+int hb_stream_sync(void* stream);
+
+/* ---- per-model kernels (CodeObject.compile / __call__) ------------------ */
+/* Compiles HIP source for gfx950 with hiprtc (-O3 -ffp-contract=off, packed
+ * fp32 off) and loads it.  `log` (may be NULL) receives the compiler log. */
+int hb_module_load(hb_ctx* ctx, const char* source, const char* extra_options, hb_module** out, char* log,
+                   size_t loglen);
+int hb_module_unload(hb_module* mod);
+int hb_module_function(hb_module* mod, const char* name, void** fn);
+/* Launches a module function with a packed argument buffer (the kernel takes
+ * one struct by value whose bytes are `args`). */
+int hb_launch(void* fn, unsigned grid, unsigned block, unsigned shmem, void* stream, const void* args,
+              size_t argsize);
+
+/* ---- step plans (Network.run hot loop + Clock.advance) ------------------ */
+/* A plan is the ordered list of launches of one time step.  `clock` points to a
+ * device int64 holding the current time step; each launch's argument buffer
+ * may hold an int64 "step offset" at byte offset `step_arg_offset` (or -1):
+ * replaying k steps patches it with 0..k-1 and then advances *clock by k. */
+int hb_plan_create(hb_ctx* ctx, int64_t* clock, hb_plan** out);
+int hb_plan_add(hb_plan* plan, void* fn, unsigned grid, unsigned block, unsigned shmem, const void* args,
+                size_t argsize, long step_arg_offset);
+/* Replaces the argument buffer of launch `index` (drops a captured graph). */
+int hb_plan_set_args(hb_plan* plan, int index, const void* args, size_t argsize);
+/* Runs n_steps steps, replaying a hipGraph of `graph_steps` steps when
+ * graph_steps > 1, ordered after prior work on `stream`.  Returns after
+ * enqueueing. */
+int hb_plan_run(hb_plan* plan, void* stream, int64_t n_steps, int graph_steps);
+int hb_plan_destroy(hb_plan* plan);
+
+/* ---- built-in kernels --------------------------------------------------- */
+/* SpikeGeneratorGroup step (spikegenerator.pyx:6-26) for the step *clock + k:
+ * emits neuron_index[lastindex .. first timebin > step) into the spike ring
+ * slot (step mod ring_slots); `state` holds {lastindex}. period_bins > 0
+ * repeats the pattern.  Launch from a plan with hb_spikegen_function. */
+int hb_spikegen_function(void** fn);
+
+/* ---- connectivity / CSR (CSpikeQueue::prepare) -------------------------- */
+/* Counts synapses per (source, delay segment): seg_count[(pre-src0)*n_seg + d-dmin]
+ * for pre in [src0, src1).  delay_steps may be NULL (scalar delay dmin). */
+int hb_csr_count(const int32_t* pre, const int32_t* delay_steps, int64_t n_syn, int32_t src0, int32_t src1,
+                 int32_t dmin, int32_t n_seg, uint32_t* seg_count, void* stream);
+/* In-place exclusive scan of n uint32 counts into n+1 offsets (last = total). */
+int hb_scan_u32(uint32_t* counts_to_offsets, int64_t n, void* stream);
+
+/* Synthetic connectivity (deterministic, counter-based SplitMix64 streams):
+ * fixed in-degree K per target, sources ascending per target. */
+int hb_gen_fixed_indegree(uint64_t seed, uint64_t stream_id, int32_t n_pre, int32_t n_post, int32_t K,
+                          int32_t* pre_of_post, int32_t* status, void* stream);
+/* CSR by (source, delay, target) from pre_of_post (n_post x K): for sources in
+ * [src0, src1) with pair delays dmin + U{0..n_seg-1} (stream delay_stream)
+ * and fp32 weights w_base * U(w_lo, w_hi) (stream w_stream).  seg_ptr has
+ * (src1-src0)*n_seg+1 entries; post/w have seg_ptr[last] entries. */
+int hb_csr_from_indegree(const int32_t* pre_of_post, int32_t n_post, int32_t K, int32_t src0, int32_t src1,
+                         uint64_t seed, uint64_t delay_stream, int32_t dmin, int32_t n_seg, uint64_t w_stream,
+                         double w_base, double w_lo, double w_hi, uint32_t* seg_ptr, uint32_t* cursor,
+                         int32_t* post, float* w, void* stream);
+/* Bernoulli(p) connectivity keyed by (i*n_post + j), CSR by source for
+ * i in [src0, src1), targets ascending: first call with post == NULL fills
+ * row_count, then (after hb_scan_u32 into row_ptr) with post != NULL. */
+int hb_gen_bernoulli_rows(uint64_t seed, uint64_t stream_id, int32_t src0, int32_t src1, int32_t n_post,
+                          double p, uint32_t* row_count_or_ptr, int32_t* post, void* stream);
+
+/* ---- host-side elementary functions (identical to the device ones) ------ */
+double hb_host_exp(double x);
+double hb_host_expm1(double x);
+double hb_host_log(double x);
+double hb_host_pow(double x, double y);
+double hb_host_exprel(double x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
