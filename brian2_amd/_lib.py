@@ -6,6 +6,15 @@ importing this module raises.
 import ctypes
 import os
 
+try:
+    # torch ships its own libamdhip64 (SONAME libamdhip64.so.7) and loads it by file
+    # name; loading torch first makes this library bind to that same HIP runtime
+    # instead of a second copy from /opt/rocm (two runtimes in one process do not
+    # share devices, streams or allocations).
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhipbrian.so")
 
@@ -16,8 +25,7 @@ class HipBrianError(RuntimeError):
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
-        f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
-        "or `python -m brian2_amd.build`"
+        f"{LIB_PATH} is missing: build it with `python brian2_amd/build.py`"
     )
 
 lib = ctypes.CDLL(LIB_PATH)
@@ -47,6 +55,7 @@ _SIGS = {
     "hb_plan_set_args": ([_vp, _i, _vp, _sz], _i),
     "hb_plan_run": ([_vp, _vp, _i64, _i], _i),
     "hb_plan_destroy": ([_vp], _i),
+    "hb_plan_profile": ([_vp, _vp, _i64, _P(ctypes.c_float)], _i),
     "hb_spikegen_function": ([_P(_vp)], _i),
     "hb_csr_count": ([_vp, _vp, _i64, _i32, _i32, _i32, _i32, _vp, _vp], _i),
     "hb_scan_u32": ([_vp, _i64, _vp], _i),

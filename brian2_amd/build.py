@@ -1,4 +1,6 @@
-"""Builds libhipbrian.so for gfx950 in place (python -m brian2_amd.build)."""
+"""Builds libhipbrian.so for gfx950 in place: `python brian2_amd/build.py`.
+
+Run as a script (it must not import the package, whose import needs the library)."""
 import os
 import subprocess
 import sys

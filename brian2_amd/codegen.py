@@ -411,6 +411,68 @@ __device__ __forceinline__ float hb_exprelf(float x) {
     if (x > 717.0f) return __builtin_inff();
     return expm1f(x) / x;
 }
+// Sorts one target's run of events by key: insertion sort for short runs, heapsort
+// (O(m log m), in place) for the long runs of synchronous bursts.
+template <typename VT>
+__device__ void hb_sort_kv(unsigned long long* k, VT* v, int n) {
+    if (n <= 24) {
+        for (int a = 1; a < n; a++) {
+            const unsigned long long kk = k[a]; const VT vv = v[a]; int b = a;
+            while (b > 0 && k[b - 1] > kk) { k[b] = k[b - 1]; v[b] = v[b - 1]; b--; }
+            k[b] = kk; v[b] = vv;
+        }
+        return;
+    }
+    for (int start = n / 2 - 1; start >= 0; start--) {
+        int r = start;
+        while (2 * r + 1 < n) {
+            int c = 2 * r + 1;
+            if (c + 1 < n && k[c] < k[c + 1]) c++;
+            if (k[r] < k[c]) { unsigned long long tk = k[r]; k[r] = k[c]; k[c] = tk; VT tv = v[r]; v[r] = v[c]; v[c] = tv; r = c; }
+            else break;
+        }
+    }
+    for (int end = n - 1; end > 0; end--) {
+        { unsigned long long tk = k[0]; k[0] = k[end]; k[end] = tk; VT tv = v[0]; v[0] = v[end]; v[end] = tv; }
+        int r = 0;
+        while (2 * r + 1 < end) {
+            int c = 2 * r + 1;
+            if (c + 1 < end && k[c] < k[c + 1]) c++;
+            if (k[r] < k[c]) { unsigned long long tk = k[r]; k[r] = k[c]; k[c] = tk; VT tv = v[r]; v[r] = v[c]; v[c] = tv; r = c; }
+            else break;
+        }
+    }
+}
+// Same ordering for a run of record indices (global path), comparing keys[idx].
+__device__ void hb_sort_idx(int* ix, const unsigned long long* keys, int n) {
+    if (n <= 24) {
+        for (int a = 1; a < n; a++) {
+            const int xx = ix[a]; const unsigned long long kk = keys[xx]; int b = a;
+            while (b > 0 && keys[ix[b - 1]] > kk) { ix[b] = ix[b - 1]; b--; }
+            ix[b] = xx;
+        }
+        return;
+    }
+    for (int start = n / 2 - 1; start >= 0; start--) {
+        int r = start;
+        while (2 * r + 1 < n) {
+            int c = 2 * r + 1;
+            if (c + 1 < n && keys[ix[c]] < keys[ix[c + 1]]) c++;
+            if (keys[ix[r]] < keys[ix[c]]) { int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
+            else break;
+        }
+    }
+    for (int end = n - 1; end > 0; end--) {
+        { int t = ix[0]; ix[0] = ix[end]; ix[end] = t; }
+        int r = 0;
+        while (2 * r + 1 < end) {
+            int c = 2 * r + 1;
+            if (c + 1 < end && keys[ix[c]] < keys[ix[c + 1]]) c++;
+            if (keys[ix[r]] < keys[ix[c]]) { int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
+            else break;
+        }
+    }
+}
 // exclusive scan of one int per thread over a workgroup of HB_T threads (HB_T/64 waves)
 __device__ __forceinline__ int hb_block_exscan(int v, int* sh, int* total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;

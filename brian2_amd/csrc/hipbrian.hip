@@ -329,6 +329,43 @@ extern "C" int hb_plan_run(hb_plan* p, void* stream, int64_t n_steps, int graph_
     return 0;
 }
 
+extern "C" int hb_plan_profile(hb_plan* p, void* stream, int64_t n_steps, float* ms_per_launch) {
+    hb_ctx* c = p->ctx;
+    HB_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const size_t nl = p->launches.size();
+    std::vector<hipEvent_t> ev(nl + 1);
+    for (auto& e : ev) HB_HIP(hipEventCreate(&e));
+    for (size_t k = 0; k < nl; k++) ms_per_launch[k] = 0.0f;
+    HB_HIP(hipEventRecord(c->ev_in, (hipStream_t)stream));
+    HB_HIP(hipStreamWaitEvent(s, c->ev_in, 0));
+    int rc = 0;
+    for (int64_t j = 0; j < n_steps && !rc; j++) {
+        HB_HIP(hipEventRecord(ev[0], s));
+        for (size_t k = 0; k < nl && !rc; k++) {
+            auto& r = p->launches[k];
+            if (r.step_off >= 0) memcpy(r.args.data() + r.step_off, &j, 8);
+            if (r.grid) rc = launch_fn(r.fn, r.grid, r.block, r.shmem, s, r.args.data(), r.args.size());
+            HB_HIP(hipEventRecord(ev[k + 1], s));
+        }
+        HB_HIP(hipEventSynchronize(ev[nl]));
+        for (size_t k = 0; k < nl; k++) {
+            float ms = 0.0f;
+            HB_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            ms_per_launch[k] += ms;
+        }
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(hb_k_clock_advance, dim3(1), dim3(64), 0, s, p->clock, n_steps);
+        HB_HIP(hipGetLastError());
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    HB_HIP(hipEventRecord(c->ev_out, s));
+    HB_HIP(hipStreamWaitEvent((hipStream_t)stream, c->ev_out, 0));
+    HB_HIP(hipStreamSynchronize(s));
+    return rc;
+}
+
 extern "C" int hb_plan_destroy(hb_plan* p) {
     if (!p) return 0;
     int rc = plan_drop_graph(p);

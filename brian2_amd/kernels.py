@@ -113,7 +113,7 @@ class NeuronKernel:
 
     def __init__(self, name, N, arrays, float_t, stateupdate=None, threshold=None, reset=None,
                  reset_mode="none", scalars=None, fold=None, bucket_cap=0, val_t="double",
-                 spikemon=None, statemon=None, flush=False, uses_refractory=False):
+                 spikemon=None, statemon=None, flush=False, uses_refractory=False, statemon_when="start"):
         self.name = name
         self.N = N
         self.arrays = arrays          # {var: ctype} of the group's state arrays
@@ -126,6 +126,7 @@ class NeuronKernel:
         self.val_t = val_t
         self.spikemon = spikemon      # (start, stop) or None
         self.statemon = statemon      # [vars] or None
+        self.statemon_when = statemon_when
         self.flush = flush
         self.uses_refractory = uses_refractory
         self.struct = None
@@ -176,6 +177,9 @@ class NeuronKernel:
             ps.ptr("bkt_cnt", "int")
             ps.ptr("bkt_key", "unsigned long long")
             ps.ptr("bkt_val", self.val_t)
+            ps.ptr("bkt_ord", "int")
+            ps.val("bkt_cap", "int")
+            ps.val("pad_bkt", "int")
         if self.reset_mode == "deferred":
             ps.ptr("spiked", "unsigned char")
         ps.ptr("status", "int")
@@ -213,22 +217,27 @@ class NeuronKernel:
         k.append("    const bool hb_active = i < N;")
         k.append("    bool hb_cond = false;")
         k.append("    (void)t; (void)dt; (void)_idx;")
-        # ---- fold: gather this tile's events into LDS, sort each target's run by key
+        # ---- fold: this tile's events, counting-sorted by target, each run sorted by key.
+        # Runs live in LDS when the tile received at most LDS_CAP events, otherwise as
+        # an index permutation in global memory (bursts).
         if self.fold:
             cap, vt = self.cap, self.val_t
             k += [
-                "    // ---- events delivered by the previous step: counting sort by target, then key order ----",
+                "    // ---- events delivered by the previous step ----",
                 f"    __shared__ unsigned long long s_key[{cap}];",
                 f"    __shared__ {vt} s_val[{cap}];",
                 f"    __shared__ int s_pos[{T}];",
                 f"    __shared__ int s_sh[{nw + 1}];",
                 "    __shared__ int s_n;",
-                f"    if (threadIdx.x == 0) {{ const int c = p.bkt_cnt[blockIdx.x]; s_n = c < {cap} ? c : {cap}; }}",
+                "    if (threadIdx.x == 0) { const int c = p.bkt_cnt[blockIdx.x]; s_n = c < p.bkt_cap ? c : p.bkt_cap; }",
                 "    s_pos[threadIdx.x] = 0;",
                 "    __syncthreads();",
                 "    const int hb_n = s_n;",
-                f"    const unsigned long long* hb_gk = p.bkt_key + (size_t)blockIdx.x * {cap};",
-                f"    const {vt}* hb_gv = p.bkt_val + (size_t)blockIdx.x * {cap};",
+                f"    const bool hb_big = hb_n > {cap};",
+                "    const size_t hb_off = (size_t)blockIdx.x * (size_t)p.bkt_cap;",
+                "    const unsigned long long* hb_gk = p.bkt_key + hb_off;",
+                f"    const {vt}* hb_gv = p.bkt_val + hb_off;",
+                "    int* hb_go = p.bkt_ord + hb_off;",
                 f"    for (int e = threadIdx.x; e < hb_n; e += {T}) atomicAdd(&s_pos[(int)(hb_gk[e] >> {KEY_TGT_SHIFT})], 1);",
                 "    __syncthreads();",
                 "    const int hb_cnt = s_pos[threadIdx.x];",
@@ -236,19 +245,23 @@ class NeuronKernel:
                 "    const int hb_beg = hb_block_exscan(hb_cnt, s_sh, &hb_tot);",
                 "    s_pos[threadIdx.x] = hb_beg;",
                 "    __syncthreads();",
-                f"    for (int e = threadIdx.x; e < hb_n; e += {T}) {{",
-                "        const unsigned long long kk = hb_gk[e];",
-                f"        const int q = atomicAdd(&s_pos[(int)(kk >> {KEY_TGT_SHIFT})], 1);",
-                "        s_key[q] = kk;",
-                "        s_val[q] = hb_gv[e];",
+                "    if (!hb_big) {",
+                f"        for (int e = threadIdx.x; e < hb_n; e += {T}) {{",
+                "            const unsigned long long kk = hb_gk[e];",
+                f"            const int q = atomicAdd(&s_pos[(int)(kk >> {KEY_TGT_SHIFT})], 1);",
+                "            s_key[q] = kk;",
+                "            s_val[q] = hb_gv[e];",
+                "        }",
+                "    } else {",
+                f"        for (int e = threadIdx.x; e < hb_n; e += {T}) {{",
+                f"            const int q = atomicAdd(&s_pos[(int)(hb_gk[e] >> {KEY_TGT_SHIFT})], 1);",
+                "            hb_go[q] = e;",
+                "        }",
                 "    }",
                 "    __syncthreads();",
                 "    if (threadIdx.x == 0) p.bkt_cnt[blockIdx.x] = 0;",
-                "    for (int a = hb_beg + 1; a < hb_beg + hb_cnt; a++) {",
-                f"        const unsigned long long kk = s_key[a]; const {vt} vv = s_val[a]; int b = a;",
-                "        while (b > hb_beg && s_key[b - 1] > kk) { s_key[b] = s_key[b - 1]; s_val[b] = s_val[b - 1]; b--; }",
-                "        s_key[b] = kk; s_val[b] = vv;",
-                "    }",
+                "    if (!hb_big) hb_sort_kv(s_key + hb_beg, s_val + hb_beg, hb_cnt);",
+                "    else hb_sort_idx(hb_go + hb_beg, hb_gk, hb_cnt);",
             ]
         # ---- per-neuron body ------------------------------------------------------
         k.append("    if (hb_active) {")
@@ -257,8 +270,11 @@ class NeuronKernel:
             body.append(f"bool {v} = p.a_{v}[i] != 0;" if ct == "bool" else f"{ct} {v} = p.a_{v}[i];")
         if self.fold:
             body.append("for (int e = hb_beg; e < hb_beg + hb_cnt; e++) {")
-            body.append(f"    const {self.val_t} hb_v = s_val[e];")
-            body.append("    switch ((int)((s_key[e] >> 44) & 15)) {")
+            body.append("    unsigned long long hb_k;")
+            body.append(f"    {self.val_t} hb_v;")
+            body.append("    if (!hb_big) { hb_k = s_key[e]; hb_v = s_val[e]; }")
+            body.append("    else { const int r = hb_go[e]; hb_k = hb_gk[r]; hb_v = hb_gv[r]; }")
+            body.append("    switch ((int)((hb_k >> 44) & 15)) {")
             for rank, var, op in self.fold:
                 body.append(f"    case {rank}: {var} {op} hb_v; break;")
             body.append("    default: break;")
@@ -268,19 +284,22 @@ class NeuronKernel:
             body.append("if (p.spiked[i]) { // reset of the previous step (after its propagation)")
             body += ["    " + x for x in self._block("rs", self.rs, {})]
             body.append("}")
+        if self.statemon and (not self.flush or self.statemon_when == "end"):
+            # when='start' records step s before its update; when='end' records the state
+            # after step s-1's propagation and reset, i.e. the same point in the loop
+            nv = len(self.statemon)
+            row = "step - p.rec_row0" if self.statemon_when == "start" else "step - 1 - p.rec_row0"
+            body += [
+                f"{{ // StateMonitor (when='{self.statemon_when}')",
+                "    int lo = 0, hi = p.n_rec;",
+                "    while (lo < hi) { const int m = (lo + hi) >> 1; if (p.rec_idx[m] < i) lo = m + 1; else hi = m; }",
+                f"    const long long row = {row};",
+                "    if (lo < p.n_rec && p.rec_idx[lo] == i && row >= 0 && row < p.rec_rows) {",
+            ]
+            for a, v in enumerate(self.statemon):
+                body.append(f"        p.rec_buf[((size_t)row * {nv} + {a}) * p.n_rec + lo] = (double){v};")
+            body += ["    }", "}"]
         if not self.flush:
-            if self.statemon:
-                nv = len(self.statemon)
-                body += [
-                    "{ // StateMonitor (when='start')",
-                    "    int lo = 0, hi = p.n_rec;",
-                    "    while (lo < hi) { const int m = (lo + hi) >> 1; if (p.rec_idx[m] < i) lo = m + 1; else hi = m; }",
-                    "    const long long row = step - p.rec_row0;",
-                    "    if (lo < p.n_rec && p.rec_idx[lo] == i && row >= 0 && row < p.rec_rows) {",
-                ]
-                for a, v in enumerate(self.statemon):
-                    body.append(f"        p.rec_buf[((size_t)row * {nv} + {a}) * p.n_rec + lo] = (double){v};")
-                body += ["    }", "}"]
             if self.su is not None:
                 body.append("{ // stateupdate")
                 body += ["    " + x for x in self._block("su", self.su, {})]
@@ -368,7 +387,7 @@ class GatherKernel:
     """Per-pathway event gather: spikes of step-d -> CSR segments of delay d -> buckets."""
 
     def __init__(self, name, rank, n_seg, dmin, on_pre, syn_arrays, scalars, val_t, bucket_cap, float_t,
-                 target_var):
+                 target_var, count_synapses=False):
         if rank >= MAX_RANK:
             raise NotImplementedError(f"more than {MAX_RANK} pathways into one group")
         if n_seg > MAX_SEG:
@@ -384,6 +403,7 @@ class GatherKernel:
         self.cap = bucket_cap
         self.float_t = float_t
         self.target_var = target_var
+        self.count_synapses = count_synapses
         self.kernel_name = f"gather_{name}"
         self._build()
 
@@ -407,7 +427,12 @@ class GatherKernel:
         ps.ptr("bkt_cnt", "int")
         ps.ptr("bkt_key", "unsigned long long")
         ps.ptr("bkt_val", self.val_t)
+        ps.val("bkt_cap", "int")
+        ps.val("pad_bkt", "int")
         ps.ptr("status", "int")
+        ps.ptr("events", "unsigned long long")  # per-workgroup delivered-event totals
+        if self.count_synapses:
+            ps.ptr("syn_count", "int")           # per-synapse event counts (CSR order)
         self.struct = ps
         T = T_TILE
         NS = self.n_seg
@@ -447,6 +472,10 @@ class GatherKernel:
         k.append(f"    if (threadIdx.x == 0) for (int s = 1; s <= {NS}; s++) s_cum[s] += s_cum[s - 1];")
         k.append("    __syncthreads();")
         k.append(f"    const int total = s_cum[{NS}];")
+        k.append("    __shared__ unsigned long long s_ev;")
+        k.append("    if (threadIdx.x == 0) s_ev = 0ull;")
+        k.append("    __syncthreads();")
+        k.append("    unsigned long long hb_ev = 0ull;")
         k.append("    const int lane = threadIdx.x & 63;")
         k.append("    const int nwaves = (gridDim.x * blockDim.x) >> 6;")
         k.append("    for (int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < total; q += nwaves) {")
@@ -459,6 +488,7 @@ class GatherKernel:
         k.append("        if (src < p.src0 || src >= p.src1) continue;")
         k.append(f"        const size_t row = (size_t)(src - p.src0) * {NS} + seg;")
         k.append("        const unsigned int a = p.seg_ptr[row], b = p.seg_ptr[row + 1];")
+        k.append("        hb_ev += (unsigned long long)(b - a);")
         k.append(f"        const unsigned long long khi = ((unsigned long long){self.rank} << 44) | "
                  f"((unsigned long long)({NS - 1} - seg) << 32);")
         k.append("        for (unsigned int e = a + lane; e < b; e += 64) {")
@@ -468,15 +498,20 @@ class GatherKernel:
         k.extend("            " + x for x in lines)
         k.append(f"            const int tile = _postsynaptic_idx / {T};")
         k.append(f"            const unsigned long long key = ((unsigned long long)(_postsynaptic_idx - tile * {T}) << {KEY_TGT_SHIFT}) | khi | (unsigned long long)e;")
+        if self.count_synapses:
+            k.append("            p.syn_count[e] += 1;")
         k.append("            const int pos = atomicAdd(&p.bkt_cnt[tile], 1);")
-        k.append(f"            if (pos < {cap}) {{")
-        k.append(f"                p.bkt_key[(size_t)tile * {cap} + pos] = key;")
-        k.append(f"                p.bkt_val[(size_t)tile * {cap} + pos] = hb_val;")
+        k.append("            if (pos < p.bkt_cap) {")
+        k.append("                p.bkt_key[(size_t)tile * (size_t)p.bkt_cap + pos] = key;")
+        k.append("                p.bkt_val[(size_t)tile * (size_t)p.bkt_cap + pos] = hb_val;")
         k.append("            } else {")
         k.append("                atomicOr(p.status, 1);")
         k.append("            }")
         k.append("        }")
         k.append("    }")
+        k.append("    if (lane == 0 && hb_ev) atomicAdd(&s_ev, hb_ev);")
+        k.append("    __syncthreads();")
+        k.append("    if (threadIdx.x == 0) p.events[blockIdx.x] += s_ev;")
         k.append("}")
         self.source = "\n".join(k) + "\n"
 

@@ -160,9 +160,11 @@ class NeuronGroup(_GroupBase):
     ``float_t`` is "double" (fp64 verification mode) or "float" (fp32 mode).
     """
 
-    def __init__(self, N, model, float_t="double", name=None, namespace=None):
+    def __init__(self, N, model=None, float_t="double", name=None, namespace=None, variables=None):
         if isinstance(model, str):
             model = Model.load(model)
+        if model is None:
+            model = Model({"group": {}})
         self.N = int(N)
         self.name = name or _unique("neurongroup")
         self.float_t = float_t
@@ -182,6 +184,8 @@ class NeuronGroup(_GroupBase):
                     owner = owner or v["owner"]
                     if v["owner"] == owner:
                         self.vars[n] = ctype_of(v["dtype"], float_t, n)
+        for n, dt_ in (variables or {}).items():  # plain state variables, e.g. "v : 1"
+            self.vars[n] = ctype_of(dt_, float_t, n)
         self.vars = dict(sorted(self.vars.items()))
         self.uses_refractory = "lastspike" in self.vars and "not_refractory" in self.vars
         self._host = {}
@@ -257,8 +261,9 @@ class Synapses:
     float64 synaptic variable as float32 on the device when every value is
     exactly representable (checked)."""
 
-    def __init__(self, source, target, model, name=None, storage=None, order=0):
+    def __init__(self, source, target, model, name=None, storage=None, order=0, count_events=False):
         self.name = name or _unique("synapses")
+        self.count_events = count_events
         self.pathway_name = f"{self.name}_pre"
         self.order = order
         if isinstance(source, Subgroup):
@@ -364,6 +369,20 @@ class Synapses:
         self.csr = dict(seg_ptr=dev(seg_ptr.astype(np.uint32).view(np.int32)), post=dev(self.post[order]),
                         syn=syn, dmin=dmin, n_seg=n_seg, n_syn=S, order=order)
 
+    @property
+    def n_events(self):
+        """Synapse ids delivered so far (Brian's 'synaptic events', synapses.pyx:42-50)."""
+        return int(self.events_dev.sum().item())
+
+    def event_counts(self):
+        """Per-synapse delivered-event counts in synapse order (count_events=True)."""
+        c = self.syn_count_dev.cpu().numpy()[: self.csr["n_syn"]]
+        if "order" in self.csr:
+            out = np.empty_like(c)
+            out[self.csr["order"]] = c
+            return out
+        return c
+
     def syn_ctypes(self):
         out = {}
         for n in self.syn_vars:
@@ -422,7 +441,10 @@ class SpikeMonitor:
 class StateMonitor:
     """Records variables at the start of every step (brian2/monitors/statemonitor.py:82)."""
 
-    def __init__(self, source, variables, record, name=None):
+    def __init__(self, source, variables, record, name=None, when="start"):
+        if when not in ("start", "end"):
+            raise NotImplementedError("StateMonitor supports when='start' and when='end'")
+        self.when = when
         self.source = source
         self.variables = list(variables) if not isinstance(variables, str) else [variables]
         rec = np.asarray(record, np.int32)
@@ -445,10 +467,13 @@ class StateMonitor:
 class Network:
     """Runs the step loop on the device (Network.run, brian2/core/network.py:1047-1292)."""
 
-    def __init__(self, *objs, dt=1e-4, bucket_cap=2048, graph_steps=100):
+    def __init__(self, *objs, dt=1e-4, bucket_cap=2048, graph_steps=100, bucket_mem_frac=0.3):
+        """bucket_cap: events per tile folded in LDS (larger bursts use global memory);
+        bucket_mem_frac: share of free device memory the per-tile event buckets may use."""
         self.objs = []
         self.dt = float(dt)
         self.bucket_cap = int(bucket_cap)
+        self.bucket_mem_frac = float(bucket_mem_frac)
         self.graph_steps = int(graph_steps)
         self.t_step = 0
         self._built = False
@@ -500,16 +525,30 @@ class Network:
                 continue
             if not isinstance(g, NeuronGroup):
                 raise NotImplementedError("pathways into a spike generator")
-            val_t = "float"
-            for s in incoming:
-                vt = self._value_type(s)
-                if vt == "double":
-                    val_t = "double"
+            vts = {self._value_type(s) for s in incoming}
+            if len(vts & {"int", "long long"}) and len(vts) > 1:
+                raise NotImplementedError("integer and floating-point pathway targets in one group")
+            val_t = "double" if "double" in vts else ("long long" if "long long" in vts else
+                                                        ("int" if "int" in vts else "float"))
             g.val_t = val_t
             g.n_tiles = (g.N + T_TILE - 1) // T_TILE
+            # Each synapse delivers at most one event per step, so a tile can never
+            # receive more events in one step than the in-degree of its targets: size
+            # every tile's bucket by the largest such in-degree (within a memory budget).
+            indeg = torch.zeros(g.n_tiles, dtype=torch.int64, device="cuda")
+            for s in incoming:
+                post = s.csr["post"][: s.csr["n_syn"]]
+                indeg += torch.bincount(torch.div(post, T_TILE, rounding_mode="floor").long(), minlength=g.n_tiles)
+            need = max(1, int(indeg.max().item()))
+            rec_bytes = 8 + (8 if val_t in ("double", "long long") else 4) + 4
+            free, _ = torch.cuda.mem_get_info()
+            budget = int(free * self.bucket_mem_frac) // (rec_bytes * g.n_tiles)
+            g.bkt_capacity = int(min(need, max(budget, 1), 2**31 - 1))
+            g.lds_cap = int(min(self.bucket_cap, g.bkt_capacity))
             g.bkt_cnt = zeros(g.n_tiles, "int")
-            g.bkt_key = torch.zeros(g.n_tiles * self.bucket_cap, dtype=torch.int64, device="cuda")
-            g.bkt_val = zeros(g.n_tiles * self.bucket_cap, val_t)
+            g.bkt_key = torch.zeros(g.n_tiles * g.bkt_capacity, dtype=torch.int64, device="cuda")
+            g.bkt_val = zeros(g.n_tiles * g.bkt_capacity, val_t)
+            g.bkt_ord = zeros(g.n_tiles * g.bkt_capacity, "int")
         self._plan = ctypes.c_void_p()
         _lib.check(L.hb_plan_create(D.ctx, ptr(self.clock), ctypes.byref(self._plan)), "hb_plan_create")
         self._launches = []
@@ -535,6 +574,8 @@ class Network:
             types[st["var"]] = ctype_of(st["dtype"], s.target.float_t, st["var"])
         _, t = Renderer(types).render(s.spec.vector[-1]["expr"])
         tv = s.target.vars[s.target_var]
+        if tv in ("int", "long long", "bool"):
+            return "long long" if tv == "long long" else "int"
         if tv == "float":
             return "float"
         return "float" if t == "float" else "double"
@@ -583,10 +624,11 @@ class Network:
                 if v not in g.vars:
                     raise KeyError(f"StateMonitor variable {v!r} is not a state variable of {g.name}")
         K = NeuronKernel(g.name, g.N, g.vars, g.float_t, g.su, g.th, g.rs, mode, scalars, fold,
-                         self.bucket_cap, getattr(g, "val_t", "double"),
+                         getattr(g, "lds_cap", 0), getattr(g, "val_t", "double"),
                          spikemon=g.smon.range if g.smon else None,
                          statemon=g.stmon.variables if g.stmon else None,
-                         uses_refractory=g.uses_refractory)
+                         uses_refractory=g.uses_refractory,
+                         statemon_when=g.stmon.when if g.stmon else "start")
         g.kernel = K
         g.fn = D.function(K.source, K.kernel_name)
         g.struct_cls = K.struct.ctypes_struct()
@@ -605,10 +647,12 @@ class Network:
         g.launch_index = len(self._launches)
         args = self._group_args(g)
         self._add_launch(g.fn, (g.N + T_TILE - 1) // T_TILE, T_TILE, args, g.struct_cls)
-        if g.incoming or mode == "deferred":
+        end_mon = g.stmon is not None and g.stmon.when == "end"
+        if g.incoming or mode == "deferred" or end_mon:
             F = NeuronKernel(g.name, g.N, g.vars, g.float_t, g.su, g.th, g.rs, mode, scalars, fold,
-                             self.bucket_cap, getattr(g, "val_t", "double"), flush=True,
-                             uses_refractory=g.uses_refractory)
+                             getattr(g, "lds_cap", 0), getattr(g, "val_t", "double"), flush=True,
+                             uses_refractory=g.uses_refractory,
+                             statemon=g.stmon.variables if end_mon else None, statemon_when="end")
             g.flush_fn = D.function(F.source, F.kernel_name)
             g.flush_struct_cls = F.struct.ctypes_struct()
         else:
@@ -624,14 +668,15 @@ class Network:
         for (tg, n), (val, ct) in g.kernel.scalars.items():
             v[f"{tg}_{n}"] = val
         if g.incoming:
-            v.update(bkt_cnt=g.bkt_cnt.data_ptr(), bkt_key=g.bkt_key.data_ptr(), bkt_val=g.bkt_val.data_ptr())
+            v.update(bkt_cnt=g.bkt_cnt.data_ptr(), bkt_key=g.bkt_key.data_ptr(), bkt_val=g.bkt_val.data_ptr(),
+                     bkt_ord=g.bkt_ord.data_ptr(), bkt_cap=g.bkt_capacity)
         if getattr(g, "spiked", None) is not None:
             v["spiked"] = g.spiked.data_ptr()
         if not flush and g.smon:
             m = g.smon
             v.update(mon_i=m.mon_i.data_ptr(), mon_step=m.mon_step.data_ptr(), mon_n=m.mon_n.data_ptr(),
                      mon_count=m.count_dev.data_ptr(), mon_cap=m.cap, mon_start=m.range[0], mon_stop=m.range[1])
-        if not flush and g.stmon:
+        if g.stmon and (not flush or g.stmon.when == "end"):
             m = g.stmon
             v.update(rec_idx=m.rec_idx.data_ptr(), rec_buf=m.rec_buf.data_ptr() if m.rec_buf is not None else 0,
                      rec_row0=getattr(m, "row0", 0), rec_rows=getattr(m, "rows", 0), n_rec=len(m.sorted_idx))
@@ -676,7 +721,7 @@ class Network:
         sc = scalar_values(s.spec, host, g.float_t)
         types_syn = s.syn_ctypes()
         K = GatherKernel(s.name, rank, s.csr["n_seg"], s.csr["dmin"], s.spec.vector, types_syn, sc, g.val_t,
-                         self.bucket_cap, g.float_t, s.target_var)
+                         self.bucket_cap, g.float_t, s.target_var, count_synapses=s.count_events)
         s.kernel = K
         s.fn = D.function(K.source, K.kernel_name)
         cls = K.struct.ctypes_struct()
@@ -685,17 +730,22 @@ class Network:
              "ring_cnt": s.source.ring_cnt.data_ptr(), "ring_stride": s.source.N, "ring_slots": s.source.ring_slots,
              "src0": s.src_range[0], "src1": s.src_range[1], "seg_ptr": s.csr["seg_ptr"].data_ptr(),
              "post": s.csr["post"].data_ptr(), "bkt_cnt": g.bkt_cnt.data_ptr(), "bkt_key": g.bkt_key.data_ptr(),
-             "bkt_val": g.bkt_val.data_ptr(), "status": self.status.data_ptr()}
+             "bkt_val": g.bkt_val.data_ptr(), "bkt_cap": g.bkt_capacity, "status": self.status.data_ptr()}
         for n in s.syn_vars:
             v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
         for n, (val, ct) in sc.items():
             v[f"c_{n}"] = val
+        nsrc = s.src_range[1] - s.src_range[0]
+        grid = int(min(2048, max(4, (nsrc * s.csr["n_seg"]) // 256)))
+        s.events_dev = _torch().zeros(grid, dtype=_torch().int64, device="cuda")
+        v["events"] = s.events_dev.data_ptr()
+        if s.count_events:
+            s.syn_count_dev = zeros(s.csr["n_syn"], "int")
+            v["syn_count"] = s.syn_count_dev.data_ptr()
         a = cls()
         for name, _ in cls._fields_:
             if name in v:
                 setattr(a, name, v[name])
-        nsrc = s.src_range[1] - s.src_range[0]
-        grid = int(min(2048, max(4, (nsrc * s.csr["n_seg"]) // 256)))
         s.launch_index = len(self._launches)
         self._add_launch(s.fn, grid, 256, a, cls)
 
@@ -757,16 +807,33 @@ class Network:
             if m is not None and n_steps > 0:
                 nv, nr = len(m.variables), len(m.sorted_idx)
                 m._rows.append(m.rec_buf.view(n_steps, nv, nr).cpu().numpy().copy())
-        self._drain_monitors()
         st = int(self.status.cpu().item())
         if st:
             raise _lib.HipBrianError(self._status_message(st))
 
+    @property
+    def n_events(self):
+        return sum(s.n_events for s in self.paths)
+
+    def profile(self, n_steps):
+        """Runs n_steps as individual launches timed with HIP events (advances the
+        simulation).  Returns [(kind, name, total_ms)] per launch of the step."""
+        if not self._built:
+            self._build()
+        ms = (ctypes.c_float * len(self._launches))()
+        _lib.check(L.hb_plan_profile(self._plan, self.device.stream, int(n_steps), ms), "hb_plan_profile")
+        self.t_step += int(n_steps)
+        out = []
+        for obj in list(self.groups) + list(self.paths):
+            kind = "gather" if isinstance(obj, Synapses) else "step"
+            out.append((kind, obj.name, float(ms[obj.launch_index])))
+        return out
+
     def _status_message(self, st):
         msg = []
         if st & 1:
-            msg.append(f"event bucket overflow (more than bucket_cap={self.bucket_cap} synaptic events for one tile of "
-                       f"{T_TILE} neurons in one step); build the Network with a larger bucket_cap")
+            msg.append(f"event bucket overflow (a tile of {T_TILE} neurons received more synaptic events in one step "
+                       f"than its bucket holds); build the Network with a larger bucket_mem_frac")
         if st & 2:
             msg.append("spike monitor buffer overflow")
         return "; ".join(msg) or f"device status {st}"

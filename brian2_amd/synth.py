@@ -239,3 +239,26 @@ def cobahh_network(N=4000, p=0.02, seed=SEED, float_t="double", monitor=True, re
         objs.append(st)
     net = Network(*objs, dt=1e-4, bucket_cap=bucket_cap, graph_steps=graph_steps)
     return dict(net=net, G=G, Ce=Ce, Ci=Ci, mon=mon, state=st)
+
+
+# ---------------------------------------------------------------------------
+# batched sweep: 1024 independent CUBA networks of uneven size
+# ---------------------------------------------------------------------------
+S_SWEEP_N, S_SWEEP_S = 7, 8
+
+
+def sweep_networks(n_nets=1024, seed=SEED, nmin=500, nmax=8000, smin=0.5, smax=2.0, model="cuba"):
+    """N_k = nmin + U{0..nmax-nmin}, 80/20 split, we_k = we * s_k with s_k
+    log-uniform in [smin, smax], connectivity seed seed + k."""
+    from .batched import SweepNetwork
+
+    m = Model.load(model)
+    we = m.pathways["exc"].variables["we"]["value"]
+    lo, hi = L.hb_host_log(smin), L.hb_host_log(smax)
+    nets = []
+    for k in range(n_nets):
+        N = nmin + int(uniform_int(rand64(seed, S_SWEEP_N, k), nmax - nmin + 1))
+        u = float(uniform(seed, S_SWEEP_S, k))
+        s = L.hb_host_exp(lo + u * (hi - lo))
+        nets.append(SweepNetwork(N, N * 4 // 5, seed + k, {"we": we * s}))
+    return nets

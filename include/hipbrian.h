@@ -17,7 +17,7 @@
  *  - hb_csr_* replace CSpikeQueue::prepare (synapses/spikequeue.h:48-105): per-source
  *    synapse lists in synapse order, with the integer delay (int)(d/dt + 0.5)
  *    (spikequeue.h:90) made a segment of each list.
- *  - hb_spikegen_step replaces the spikegenerator template
+ *  - hb_spikegen_function's kernel replaces the spikegenerator template
  *    (codegen/runtime/cython_rt/templates/spikegenerator.pyx:6-26).
  */
 #ifndef HIPBRIAN_H
@@ -83,6 +83,10 @@ int hb_plan_set_args(hb_plan* plan, int index, const void* args, size_t argsize)
  * graph_steps > 1, ordered after prior work on `stream`.  Returns after
  * enqueueing. */
 int hb_plan_run(hb_plan* plan, void* stream, int64_t n_steps, int graph_steps);
+/* Runs n_steps steps as individual launches with a HIP event after each one, on
+ * the plan's stream, and accumulates each launch's duration over the steps into
+ * ms_per_launch[index] (synchronous). */
+int hb_plan_profile(hb_plan* plan, void* stream, int64_t n_steps, float* ms_per_launch);
 int hb_plan_destroy(hb_plan* plan);
 
 /* ---- built-in kernels --------------------------------------------------- */

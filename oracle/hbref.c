@@ -464,7 +464,7 @@ int hbr_run(hbr_sim* S) {
         rt[gi].spikes = (int32_t*)malloc(sizeof(int32_t) * (size_t)(g->N > 0 ? g->N : 1));
         if (g->model == HBR_CUBA_EXACT) cuba_scalar(g->params, dt, &rt[gi].cuba);
         else if (g->model == HBR_COBAHH_EE) hh_scalar(g->params, dt, &rt[gi].hh);
-        else if (g->model != HBR_SPIKEGEN) {
+        else if (g->model != HBR_SPIKEGEN && g->model != HBR_PASSIVE) {
             free(rt);
             return fail("unknown model");
         }
@@ -507,6 +507,7 @@ int hbr_run(hbr_sim* S) {
             hbr_group* g = &S->groups[gi];
             group_rt* R = &rt[gi];
             R->nspk = 0;
+            if (g->model == HBR_PASSIVE) continue; /* "v : 1": no threshold */
             if (g->model == HBR_SPIKEGEN) {
                 int32_t timebin = (int32_t)step; /* t_in_timesteps */
                 int64_t idx;

@@ -41,7 +41,7 @@ int64_t hbr_queue_size(hbr_queue* q);
 int32_t hbr_queue_offset(hbr_queue* q);
 
 /* ---- network step restatement ---------------------------------------------- */
-enum { HBR_CUBA_EXACT = 1, HBR_COBAHH_EE = 2, HBR_LIF_EXACT = 3, HBR_SPIKEGEN = 4 };
+enum { HBR_CUBA_EXACT = 1, HBR_COBAHH_EE = 2, HBR_LIF_EXACT = 3, HBR_SPIKEGEN = 4, HBR_PASSIVE = 5 };
 
 typedef struct {
     int32_t model;          /* HBR_* */

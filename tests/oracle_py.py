@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(ORACLE_DIR, "libhbref.so")
 i32, i64, u64, dbl, u8 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint8
 P = ctypes.POINTER
 
-HBR_CUBA_EXACT, HBR_COBAHH_EE, HBR_LIF_EXACT, HBR_SPIKEGEN = 1, 2, 3, 4
+HBR_CUBA_EXACT, HBR_COBAHH_EE, HBR_LIF_EXACT, HBR_SPIKEGEN, HBR_PASSIVE = 1, 2, 3, 4, 5
 
 
 class Group(ctypes.Structure):

@@ -1,0 +1,134 @@
+"""Code generation (CPU only): expression rendering, host evaluation of the
+loop-invariant block, and compilation of the generated kernels for gfx950."""
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from brian2_amd import codegen as cg
+from brian2_amd.kernels import GatherKernel, NeuronKernel, scalar_values
+from brian2_amd.runtime import Model, NeuronGroup
+
+
+def R(expr, **types):
+    return cg.Renderer(types).render(expr)
+
+
+def test_render_association_and_types():
+    c, t = R("_lio_12 + (((_lio_13 * ge) + (_lio_14 * gi)) + (_lio_15 * v))", _lio_12="double", _lio_13="double",
+             _lio_14="double", _lio_15="double", ge="double", gi="double", v="double")
+    assert c == "(_lio_12 + (((_lio_13 * ge) + (_lio_14 * gi)) + (_lio_15 * v)))" and t == "double"
+    # left association of a chain is kept (a - b) - c
+    assert R("a - b - c", a="double", b="double", c="double")[0] == "((a - b) - c)"
+
+
+def test_render_division_and_powers():
+    assert R("a / b", a="int", b="int") == ("((double)(a) / (double)(b))", "double")
+    assert R("a / b", a="double", b="int") == ("(a / b)", "double")
+    assert R("n ** 4", n="double") == ("hb_pow((double)(n), (double)(4))", "double")
+    assert R("x ** 2", x="float")[1] == "float"
+    assert R("a // b", a="int", b="int")[0] == "hb_floordiv<int>(a, b)"
+    assert R("a % b", a="int", b="int")[0] == "hb_imod<int>(a, b)"
+
+
+def test_render_booleans_and_functions():
+    assert R("(v > Vt) and not_refractory", v="double", Vt="double", not_refractory="bool") == \
+        ("((v > Vt) && not_refractory)", "bool")
+    assert R("not a or b", a="bool", b="bool")[0] == "((!a) || b)"
+    assert R("timestep(t - lastspike, dt) >= _lio_1", t="double", lastspike="double", dt="double",
+             _lio_1="long long")[0] == "(hb_timestep((double)((t - lastspike)), (double)(dt)) >= _lio_1)"
+    assert R("exprel(x)", x="double")[0] == "hb_exprel((double)(x))"
+    assert R("exp(x)", x="float")[0] == "expf(x)"
+    assert R("int(b)", b="bool") == ("((int)(b))", "int")
+    with pytest.raises(NotImplementedError):
+        R("rand()")
+
+
+def test_literals_round_trip():
+    for v in (0.1, 1e-16, 2980.9579870417283, -0.05555555555555555, 1.0, 5e-324):
+        assert float(cg.c_literal(v)) == v
+
+
+def test_host_eval_matches_c_semantics():
+    ev = cg.HostEval({"dt": 1e-4, "taum": 0.02})
+    assert ev.eval("timestep(0.005, dt)") == 50
+    assert ev.eval("(- dt) / taum") == -1e-4 / 0.02
+    assert ev.eval("7 / 2") == 3.5
+    assert ev.eval("2 ** 10") == 1024.0
+
+
+def test_translate_statement_bool_split_and_conditional_write():
+    st = {"var": "_v", "op": ":=", "expr": "x", "dtype": "float64", "bool_vars": ["nr"],
+          "bool_exprs": [[[["nr", False]], "a + v"], [[["nr", True]], "b + v"]]}
+    types = {"a": "double", "b": "double", "v": "double", "nr": "bool"}
+    lines = cg.translate_statement(st, types, set(), "double")
+    assert lines == ["double _v;", "if (!nr) { _v = (double)((a + v)); }", "else  { _v = (double)((b + v)); }"]
+    out = cg.translate_block([{"var": "v", "op": "=", "expr": "_v", "dtype": "float64"}], {"v": "nr"},
+                             dict(types, _v="double"), set(), "double")
+    assert out[0] == "if (nr) {" and out[-1] == "}"
+
+
+def _scalars(G, dt=1e-4):
+    sc = {}
+    for tag, spec in (("su", G.su), ("th", G.th), ("rs", G.rs)):
+        if spec is None:
+            continue
+        for n, vc in scalar_values(spec, {"dt": dt, "N": G.N}, G.float_t).items():
+            sc[(tag, n)] = vc
+    return sc
+
+
+def test_cuba_loop_invariants_equal_the_oracles():
+    """The host-evaluated _lio values are the same doubles the oracle computes."""
+    import math
+
+    G = NeuronGroup(10, Model.load("cuba"))
+    sc = _scalars(G)
+    dt, taum, taue, taui, El = 1e-4, 0.02, 0.005, 0.01, -0.049
+    from brian2_amd._lib import lib as L
+
+    e = L.hb_host_exp
+    assert sc[("su", "_lio_1")] == (50, "long long")
+    assert sc[("su", "_lio_2")][0] == e((-dt) / taue)
+    assert sc[("su", "_lio_12")][0] == El - (El * e((-dt) / taum))
+    lio4, lio5 = taue * e((-dt) / taue), -e(dt / taue)
+    assert sc[("su", "_lio_13")][0] == ((lio4 * (lio5 + e(dt / taum))) * e((-dt) / taum)) / ((0.0 - taum) + taue)
+    assert sc[("th", "Vt")][0] == -0.05 and sc[("rs", "Vr")][0] == -0.06
+    assert not math.isnan(sc[("su", "_lio_14")][0])
+
+
+def _compile(source):
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR")) as d:
+        src = os.path.join(d, "k.hip")
+        with open(src, "w") as f:
+            f.write("#include <hip/hip_runtime.h>\n" + source)
+        r = subprocess.run(["hipcc", "--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-c", src, "-o",
+                            os.path.join(d, "k.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-4000:]
+
+
+def _no_float_atomics(source):
+    for m in re.finditer(r"atomic\w+\(([^,]+),", source):
+        arg = m.group(1)
+        assert "val" not in arg and "a_" not in arg, f"floating-point atomic on {arg}"
+
+
+@pytest.mark.parametrize("model,float_t", [("cuba", "double"), ("cobahh", "double"), ("cuba_synw", "float")])
+def test_generated_kernels_compile_for_gfx950(model, float_t):
+    m = Model.load(model)
+    G = NeuronGroup(1000, m, float_t=float_t)
+    sc = _scalars(G)
+    fold = [(0, "gi", "+="), (1, "ge", "+=")]
+    K = NeuronKernel("g", 1000, G.vars, float_t, G.su, G.th, G.rs, "fused" if G.rs else "none", sc, fold, 1024,
+                     "double", spikemon=(0, 1000), statemon=["v"], uses_refractory=G.uses_refractory)
+    _no_float_atomics(K.source)
+    _compile(K.source)
+    spec = m.pathways["exc"]
+    syn = {"w": "float"} if model == "cuba_synw" else {}
+    gk = GatherKernel("p", 1, 50 if model == "cuba_synw" else 1, 1, spec.vector, syn,
+                      scalar_values(spec, {"dt": 1e-4}, float_t), "double", 1024, float_t, "ge")
+    _no_float_atomics(gk.source)
+    _compile(gk.source)

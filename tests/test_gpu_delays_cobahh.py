@@ -1,0 +1,92 @@
+"""Delays + per-synapse fp32 weights, and COBAHH (exponential_euler), on the device
+vs Brian's cython runtime (golden fixtures) and the oracle."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_synw_device_csr_matches_oracle_generator():
+    """Device fixed in-degree sampling + (source, delay) CSR + fp32 weights == the oracle's generator."""
+    from brian2_amd import synth
+
+    z, meta = O.load_golden("cuba_delay")
+    c = synth.cuba_synw_network(N=meta["N"], K=meta["K"], dmin=meta["dmin"], dmax=meta["dmax"], seed=meta["seed"],
+                                ginh=meta["ginh"])
+    pre, post, d, w32, exc = O.cuba_delay_synapses(meta)
+    n_seg = meta["dmax"] - meta["dmin"] + 1
+    for S, sel in ((c["Ce"], exc), (c["Ci"], ~exc)):
+        a = S.src_range[0]
+        seg = S.csr["seg_ptr"].cpu().numpy().view(np.uint32).astype(np.int64)
+        dpost = S.csr["post"].cpu().numpy()
+        dw = S.csr["syn"]["w"].cpu().numpy()[: len(dpost)]
+        counts = np.diff(seg)
+        rows = np.repeat(np.arange(len(counts)), counts)
+        dpre = a + rows // n_seg
+        ddel = meta["dmin"] + rows % n_seg
+        # oracle order: (pre, delay, post)
+        o = np.lexsort((post[sel], d[sel], pre[sel]))
+        assert np.array_equal(dpre, pre[sel][o])
+        assert np.array_equal(ddel, d[sel][o])
+        assert np.array_equal(dpost, post[sel][o])
+        assert np.array_equal(dw, w32[sel][o])
+
+
+def test_synw_delays_bitexact_vs_brian():
+    from brian2_amd import synth
+
+    z, meta = O.load_golden("cuba_delay")
+    c = synth.cuba_synw_network(N=meta["N"], K=meta["K"], dmin=meta["dmin"], dmax=meta["dmax"], seed=meta["seed"],
+                                ginh=meta["ginh"], record=[0, 1, 2, 3])
+    c["net"].run(meta["n_steps"])
+    assert np.array_equal(c["mon"].i, z["i"]) and np.array_equal(c["mon"].step, z["step"])
+    final = np.stack([c["G"].get(n) for n in ("v", "ge", "gi")])
+    assert np.array_equal(final, z["final"])
+    tr = c["state"].values()[:, :, : z["trace"].shape[2]]
+    assert np.array_equal(tr, z["trace"])
+
+
+def test_synw_determinism_run_twice():
+    from brian2_amd import synth
+
+    z, meta = O.load_golden("cuba_delay")
+    outs = []
+    for _ in range(2):
+        c = synth.cuba_synw_network(N=meta["N"], K=meta["K"], seed=meta["seed"], ginh=meta["ginh"])
+        c["net"].run(600)
+        outs.append((c["mon"].i, c["mon"].step, c["G"].get("v"), c["G"].get("ge"), c["G"].get("gi")))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_cobahh_bitexact_vs_brian():
+    from brian2_amd import synth
+
+    z, meta = O.load_golden("cobahh4000")
+    c = synth.cobahh_network(N=meta["N"], p=meta["p"], seed=meta["seed"], record=[0, 1, 2, 3])
+    assert np.array_equal(c["G"].get("v")[[0, 1, 2, 3]], z["trace"][0, :, 0])
+    c["net"].run(meta["n_steps"])
+    tr = c["state"].values()[:, :, : z["trace"].shape[2]]
+    assert np.array_equal(tr, z["trace"])
+    assert np.array_equal(c["mon"].i, z["i"]) and np.array_equal(c["mon"].step, z["step"])
+    names = ["v", "m", "n", "h", "ge", "gi"]
+    final = np.stack([c["G"].get(n) for n in names])
+    assert np.array_equal(final, z["final"])
+
+
+def test_cobahh_fp32_error_vs_fp64():
+    """fp32 mode: error against the fp64 oracle over 100 ms (reported, bounded)."""
+    from brian2_amd import synth
+
+    z, meta = O.load_golden("cobahh4000")
+    c = synth.cobahh_network(N=meta["N"], p=meta["p"], seed=meta["seed"], float_t="float", record=[0, 1, 2, 3])
+    c["net"].run(1000)
+    v32 = c["state"].values()[0]
+    v64 = z["trace"][0, :, :1000]
+    err = np.max(np.abs(v32 - v64))
+    n64 = int((z["step"] < 1000).sum())
+    n32 = len(c["mon"].i)
+    print(f"COBAHH fp32 vs fp64 over 1000 steps: max |dv| = {err:.3e} V, spikes {n32} vs {n64}")
+    assert abs(n32 - n64) <= 0.05 * n64 + 5
