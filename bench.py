@@ -76,7 +76,7 @@ def main():
     log(f"device: {torch.cuda.get_device_name(0)}")
     t0 = time.perf_counter()
     c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=args.graph_steps,
-                                bucket_cap=2048)
+                                bucket_cap=None)
     torch.cuda.synchronize()
     log(f"built network: {c['n_syn']} synapses in {time.perf_counter() - t0:.1f} s")
     net = c["net"]

@@ -444,10 +444,10 @@ __device__ void hb_sort_kv(unsigned long long* k, VT* v, int n) {
     }
 }
 // Same ordering for a run of record indices (global path), comparing keys[idx].
-__device__ void hb_sort_idx(int* ix, const unsigned long long* keys, int n) {
+__device__ void hb_sort_idx(unsigned int* ix, const unsigned long long* keys, int n) {
     if (n <= 24) {
         for (int a = 1; a < n; a++) {
-            const int xx = ix[a]; const unsigned long long kk = keys[xx]; int b = a;
+            const unsigned int xx = ix[a]; const unsigned long long kk = keys[xx]; int b = a;
             while (b > 0 && keys[ix[b - 1]] > kk) { ix[b] = ix[b - 1]; b--; }
             ix[b] = xx;
         }
@@ -458,17 +458,17 @@ __device__ void hb_sort_idx(int* ix, const unsigned long long* keys, int n) {
         while (2 * r + 1 < n) {
             int c = 2 * r + 1;
             if (c + 1 < n && keys[ix[c]] < keys[ix[c + 1]]) c++;
-            if (keys[ix[r]] < keys[ix[c]]) { int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
+            if (keys[ix[r]] < keys[ix[c]]) { unsigned int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
             else break;
         }
     }
     for (int end = n - 1; end > 0; end--) {
-        { int t = ix[0]; ix[0] = ix[end]; ix[end] = t; }
+        { unsigned int t = ix[0]; ix[0] = ix[end]; ix[end] = t; }
         int r = 0;
         while (2 * r + 1 < end) {
             int c = 2 * r + 1;
             if (c + 1 < end && keys[ix[c]] < keys[ix[c + 1]]) c++;
-            if (keys[ix[r]] < keys[ix[c]]) { int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
+            if (keys[ix[r]] < keys[ix[c]]) { unsigned int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
             else break;
         }
     }

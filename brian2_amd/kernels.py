@@ -5,36 +5,37 @@ Step semantics follow Brian's schedule (brian2/core/network.py:42-50,
 per pathway -> reset -> clock.advance.  On the device one step is
 
   neuron kernel (per group)   fold of the events delivered by propagate(step-1)
-                              [deferred reset of step-1] [StateMonitor, 'start']
+                              [deferred reset of step-1] [StateMonitor]
                               stateupdate; threshold; [fused reset];
                               spike compaction into the ring slot of `step`;
                               [SpikeMonitor]
-  gather kernel (per pathway) events of `step` (spikes of step-d whose synapses
-                              have delay d, d in [dmin, dmax]) appended to the
-                              target group's per-tile buckets
+  gather kernel (per pathway) events of `step`: spikes of step-d whose synapses
+                              have delay d, d in [dmin, dmax] (the spike queue as
+                              a ring of spike lists indexed by integer delay)
 
-The fold of a target reproduces the cython runtime's sequential left fold in
-queue order (brian2/codegen/runtime/cython_rt/templates/synapses.pyx:42-50):
-each event carries the key (pathway rank, dmax - delay, CSR position), which is
+Event staging.  The gather works on chunks of 256 (delay, spike) pairs.  A
+workgroup flattens the chunk's synapse segments into events, counts them per
+target tile in LDS, reserves space for the whole chunk with one global atomic,
+and writes the events sorted by tile (a "run") plus one row of per-tile offsets
+into a run table.  Only two global atomics are issued per chunk; no event is
+appended with an atomic.
+
+Fold.  The neuron kernel of a tile reads its slice of every run, counting-sorts
+the events by target in LDS, and every target sorts its own events by key before
+applying them one by one.  The key (pathway rank, dmax - delay, CSR position) is
 ordered exactly like (pathway in schedule order, push step, source index,
-synapse index), and every target sorts its own events by that key before it
-applies them one by one.  No floating-point atomics are used.
+synapse index), i.e. like the cython runtime's sequential left fold over the
+spike queue (brian2/codegen/runtime/cython_rt/templates/synapses.pyx:42-50).
+No floating-point atomics are used, so the sums are bit-identical to Brian's.
 """
-import ctypes
+import re
 
-from .codegen import (
-    ParamStruct,
-    Renderer,
-    c_literal,
-    ctype_of,
-    source_header,
-    translate_block,
-)
+from .codegen import ParamStruct, Renderer, c_literal, ctype_of, source_header, translate_block
 
-T_TILE = 256        # targets (neurons) per workgroup of the neuron kernel
 KEY_TGT_SHIFT = 48  # event key: [63:48] target in tile, [47:44] rank, [43:32] delay, [31:0] CSR position
 MAX_RANK = 16
 MAX_SEG = 4096
+GATHER_BLOCK = 1024
 
 
 class CodeSpec:
@@ -53,14 +54,6 @@ class CodeSpec:
         self.vector_write = set(b.get("vector_write", []))
         self.variables = d["variables"]
         self.indices = d.get("indices", {})
-
-    def to_dict(self):
-        return {"name": self.name, "template": self.template, "owner": self.owner,
-                "blocks": {"None": {"scalar": self.scalar, "vector": self.vector,
-                                    "conditional_write": self.conditional_write,
-                                    "vector_read": sorted(self.vector_read),
-                                    "vector_write": sorted(self.vector_write)}},
-                "variables": self.variables, "indices": self.indices}
 
 
 def identifiers(statements):
@@ -107,13 +100,201 @@ def scalar_values(spec, host_values, float_t):
     return out
 
 
+def _rename(lines, names):
+    """Whole-identifier substitution of scalar names by their struct fields."""
+    if not names:
+        return lines
+    pat = re.compile(r"(?<![\w.])(" + "|".join(re.escape(k) for k in sorted(names, key=len, reverse=True))
+                     + r")(?!\w)")
+    return [pat.sub(lambda m: names[m.group(1)], ln) for ln in lines]
+
+
+FOLD_EVENTS_RESIDENT = 4096  # fold: events of a tile kept in registers (spread over its threads)
+GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per window
+
+
+def fold_capacity(T, val_t):
+    """Events of one tile sorted in LDS at a time, sized so that two workgroups of
+    T threads fit the 160 KiB of LDS of one CU."""
+    vbytes = 8 if val_t in ("double", "long long") else 4
+    cap = (81920 - 16 * T - 1024) // (8 + vbytes)
+    return int(min(8192, cap - cap % 64))
+
+
+def _fold_source(T, cap, vt, flush):
+    """Fold of the events staged by the previous step's gathers.
+
+    Phase A reads this tile's slice of every run (keys, values and positions kept
+    in registers when they fit) and counts events per target.  Phase B assigns the
+    targets to passes: all of them at once when the tile's events fit in LDS, else
+    by the position of their first event in windows of cap/2 (a pass then covers
+    at most cap events).  In each pass the events of its targets are scattered into
+    LDS, and the thread of each target sorts its run by key and applies it to the
+    target's registers.  A target with more than cap/2 events in one step (an
+    in-degree above cap/2) is sorted through an index list in global memory."""
+    nw = T // 64
+    U = max(1, min(8, FOLD_EVENTS_RESIDENT // T))
+    W = cap // 2
+    k = [
+        "    // ---- events delivered by propagate(step - 1): this tile's slice of every run ----",
+        "    const int hb_par = (int)(step & 1), hb_ppar = (int)((step - 1) & 1);",
+        "    (void)hb_par;",
+        f"    __shared__ unsigned long long s_key[{cap}];",
+        f"    __shared__ {vt} s_val[{cap}];",
+        f"    __shared__ unsigned int s_lo[{T}];",
+        f"    __shared__ int s_ro[{T}];",
+        f"    __shared__ int s_beg[{T}];",
+        f"    __shared__ int s_fill[{T}];",
+        f"    __shared__ int s_sh[{nw + 1}];",
+        "    __shared__ int s_R;",
+        "    if (threadIdx.x == 0) {",
+        "        const int r = p.run_rows[hb_ppar];",
+        "        s_R = r < p.rows_cap ? r : p.rows_cap;",
+    ]
+    if not flush:
+        k += ["        // the counters of this step's gathers (read by the next step's fold)",
+              "        if (blockIdx.x == 0) { p.run_rows[hb_par] = 0; p.stg_top[hb_par] = 0ull; }"]
+    k += [
+        "    }",
+        "    s_fill[threadIdx.x] = 0;",
+        "    __syncthreads();",
+        "    const int hb_R = s_R;",
+        "    const size_t hb_rs = (size_t)p.n_tiles + 1;",
+        f"    unsigned long long hb_fk[{U}];",
+        f"    {vt} hb_fv[{U}];",
+        f"    unsigned int hb_fq[{U}];",
+        "    int hb_nres = -1;  // events held in registers (-1: re-read them from the runs)",
+        "    int hb_tot = 0;",
+        # row batch loader (sets s_lo/s_ro, returns the batch's events and row count)
+        "#define HB_ROWS(r0, tot, nr) { \\",
+        "        const int r_ = (r0) + threadIdx.x; \\",
+        "        unsigned int lo_ = 0, len_ = 0; \\",
+        "        if (r_ < hb_R) { lo_ = p.run_tab[r_ * hb_rs + blockIdx.x]; "
+        "len_ = p.run_tab[r_ * hb_rs + blockIdx.x + 1] - lo_; } \\",
+        "        const int ex_ = hb_block_exscan((int)len_, s_sh, &(tot)); \\",
+        "        s_lo[threadIdx.x] = lo_; s_ro[threadIdx.x] = ex_; \\",
+        f"        nr = hb_R - (r0) < {T} ? hb_R - (r0) : {T}; \\",
+        "        __syncthreads(); }",
+        "#define HB_LOCATE(ev, nr, pos) { \\",
+        "        int a_ = 0, b_ = (nr); \\",
+        "        while (b_ - a_ > 1) { const int m_ = (a_ + b_) >> 1; if (s_ro[m_] <= (ev)) a_ = m_; else b_ = m_; } \\",
+        "        pos = s_lo[a_] + (unsigned int)((ev) - s_ro[a_]); }",
+        "    // phase A: events per target of this tile",
+        f"    for (int r0 = 0; r0 < hb_R; r0 += {T}) {{",
+        "        int tot, nr;",
+        "        HB_ROWS(r0, tot, nr)",
+        f"        const bool res = r0 == 0 && hb_R <= {T} && tot <= {T * U};",
+        f"        for (int b = 0; b < tot; b += {T * U}) {{",
+        "            // positions first, then all loads in flight at once, then the counts",
+        "            #pragma unroll",
+        f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+        f"                const int ev = b + hb_u * {T} + (int)threadIdx.x;",
+        "                const int evc = ev < tot ? ev : tot - 1;",
+        "                HB_LOCATE(evc, nr, hb_fq[hb_u])",
+        "            }",
+        "            #pragma unroll",
+        f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+        "                hb_fk[hb_u] = p.stg_key[hb_fq[hb_u]];",
+        "                if (res) hb_fv[hb_u] = p.stg_val[hb_fq[hb_u]];",
+        "            }",
+        "            #pragma unroll",
+        f"            for (int hb_u = 0; hb_u < {U}; hb_u++)",
+        f"                if (b + hb_u * {T} + (int)threadIdx.x < tot) atomicAdd(&s_fill[(int)(hb_fk[hb_u] >> {48})], 1);",
+        "        }",
+        "        if (res) hb_nres = tot;",
+        "        hb_tot += tot;",
+        "        __syncthreads();",
+        "    }",
+        "    // phase B: first event of every target; passes",
+        "    const int hb_c = s_fill[threadIdx.x];",
+        "    int hb_tot2;",
+        "    const int hb_b = hb_block_exscan(hb_c, s_sh, &hb_tot2);",
+        f"    const bool hb_one = hb_tot <= {cap};",
+        f"    const bool hb_bigt = !hb_one && hb_c > {W};",
+        f"    const int hb_np = hb_one ? (hb_tot > 0 ? 1 : 0) : (hb_tot + {W - 1}) / {W};",
+        f"    const int hb_myp = hb_one ? 0 : hb_b / {W};",
+        "    s_beg[threadIdx.x] = hb_bigt ? ~hb_b : hb_b;",
+        "    s_fill[threadIdx.x] = 0;",
+        "    const bool hb_anybig = __syncthreads_or(hb_bigt) != 0;",
+        "    // bursts beyond the tile's index list (only when memory capped it) are reported, not folded",
+        "    const bool hb_ovf = hb_anybig && (long long)hb_tot > p.tile_cap;",
+        "    if (hb_ovf && threadIdx.x == 0) atomicOr(p.status, 4);",
+        "    unsigned int* hb_go = p.stg_ord + (size_t)blockIdx.x * (size_t)p.tile_cap;",
+        "#define HB_SCATTER(kk, vv, pos) { \\",
+        f"        const int t_ = (int)((kk) >> {48}); \\",
+        "        const int bt_ = s_beg[t_]; \\",
+        "        if (bt_ >= 0) { \\",
+        f"            if (hb_one || bt_ / {W} == hk_pass) {{ \\",
+        "                const int q_ = bt_ + atomicAdd(&s_fill[t_], 1) - hk_pb; \\",
+        "                s_key[q_] = (kk); s_val[q_] = (vv); } \\",
+        "        } else if (hk_pass == 0 && !hb_ovf) hb_go[~bt_ + atomicAdd(&s_fill[t_], 1)] = (pos); }",
+    ]
+    return k, U, W
+
+
+def _fold_passes(T, U, W, cap, vt, apply_lines, apply_global_lines):
+    """Phase C of the fold (see _fold_source): passes over target ranges."""
+    k = [
+        "    for (int hk_pass = 0; hk_pass < hb_np; hk_pass++) {",
+        f"        const int hk_pb = hb_one ? 0 : hk_pass * {W};",
+        "        if (hb_nres >= 0) {",
+        "            #pragma unroll",
+        f"            for (int hb_u = 0; hb_u < {U}; hb_u++)",
+        f"                if (hb_u * {T} + (int)threadIdx.x < hb_nres) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], hb_fq[hb_u])",
+        "        } else {",
+        f"            for (int r0 = 0; r0 < hb_R; r0 += {T}) {{",
+        "                int tot, nr;",
+        "                HB_ROWS(r0, tot, nr)",
+        f"                for (int ev = threadIdx.x; ev < tot; ev += {T}) {{",
+        "                    unsigned int pos;",
+        "                    HB_LOCATE(ev, nr, pos)",
+        "                    const unsigned long long kk = p.stg_key[pos];",
+        f"                    const {vt} vv = p.stg_val[pos];",
+        "                    HB_SCATTER(kk, vv, pos)",
+        "                }",
+        "                __syncthreads();",
+        "            }",
+        "        }",
+        "        __syncthreads();",
+        "        if (!hb_bigt && hb_myp == hk_pass && hb_c > 0) {",
+        "            const int o = hb_b - hk_pb;",
+        "            hb_sort_kv(s_key + o, s_val + o, hb_c);",
+        "            for (int e = o; e < o + hb_c; e++) {",
+        "                const unsigned long long hb_k = s_key[e];",
+        f"                const {vt} hb_v = s_val[e];",
+    ]
+    k += ["                " + x for x in apply_lines]
+    k += [
+        "            }",
+        "        }",
+        "        __syncthreads();",
+        "    }",
+        "    if (hb_bigt && !hb_ovf) {",
+        "        hb_sort_idx(hb_go + hb_b, p.stg_key, hb_c);",
+        "        for (int e = hb_b; e < hb_b + hb_c; e++) {",
+        "            const unsigned int r = hb_go[e];",
+        "            const unsigned long long hb_k = p.stg_key[r];",
+        f"            const {vt} hb_v = p.stg_val[r];",
+    ]
+    k += ["            " + x for x in apply_global_lines]
+    k += [
+        "        }",
+        "    }",
+        "#undef HB_ROWS",
+        "#undef HB_LOCATE",
+        "#undef HB_SCATTER",
+    ]
+    return k
+
+
 # ---------------------------------------------------------------------------
 class NeuronKernel:
     """Fused per-group step kernel (or the end-of-run flush kernel)."""
 
     def __init__(self, name, N, arrays, float_t, stateupdate=None, threshold=None, reset=None,
-                 reset_mode="none", scalars=None, fold=None, bucket_cap=0, val_t="double",
-                 spikemon=None, statemon=None, flush=False, uses_refractory=False, statemon_when="start"):
+                 reset_mode="none", scalars=None, fold=None, lds_cap=0, val_t="double",
+                 spikemon=None, statemon=None, flush=False, uses_refractory=False, statemon_when="start",
+                 tile=256):
         self.name = name
         self.N = N
         self.arrays = arrays          # {var: ctype} of the group's state arrays
@@ -122,42 +303,29 @@ class NeuronKernel:
         self.reset_mode = reset_mode  # 'fused' | 'deferred' | 'none'
         self.scalars = scalars or {}  # {(tag, name): (value, ctype)}
         self.fold = fold or []        # [(rank, var, op)]
-        self.cap = bucket_cap
         self.val_t = val_t
+        self.cap = fold_capacity(tile, val_t) if not lds_cap else int(lds_cap)
+        if self.cap < 128 or self.cap % 2:
+            raise ValueError("fold capacity must be an even number of at least 128 events")
         self.spikemon = spikemon      # (start, stop) or None
         self.statemon = statemon      # [vars] or None
         self.statemon_when = statemon_when
         self.flush = flush
         self.uses_refractory = uses_refractory
-        self.struct = None
-        self.source = None
+        self.T = tile
         self._build()
 
-    # -- names used by the vector code of one code object ------------------
     def _block(self, tag, spec, extra_types):
-        types = {}
-        names = {}
-        for v, ct in self.arrays.items():
-            types[v] = ct
-        types["t"] = "double"
-        types["dt"] = "double"
-        types["N"] = "int"
-        types["i"] = "int"
-        types["_idx"] = "int"
+        types = dict(self.arrays)
+        types.update({"t": "double", "dt": "double", "N": "int", "i": "int", "_idx": "int"})
         types.update(extra_types)
+        names = {}
         for (tg, n), (_, ct) in self.scalars.items():
             if tg == tag:
                 types[n] = ct
                 names[n] = f"p.{tag}_{n}"
-        declared = set()
-        lines = translate_block(spec.vector, spec.conditional_write, types, declared, self.float_t)
-        # rename scalar names to struct fields (whole-identifier substitution)
-        import re
-
-        if names:
-            pat = re.compile(r"(?<![\w.])(" + "|".join(re.escape(k) for k in sorted(names, key=len, reverse=True)) + r")(?!\w)")
-            lines = [pat.sub(lambda m: names[m.group(1)], ln) for ln in lines]
-        return lines
+        lines = translate_block(spec.vector, spec.conditional_write, types, set(), self.float_t)
+        return _rename(lines, names)
 
     def _build(self):
         ps = ParamStruct(f"hbp_{self.name}")
@@ -174,12 +342,15 @@ class NeuronKernel:
         ps.ptr("ring_cnt", "int")
         ps.val("ring_stride", "long long")
         if self.fold:
-            ps.ptr("bkt_cnt", "int")
-            ps.ptr("bkt_key", "unsigned long long")
-            ps.ptr("bkt_val", self.val_t)
-            ps.ptr("bkt_ord", "int")
-            ps.val("bkt_cap", "int")
-            ps.val("pad_bkt", "int")
+            ps.ptr("stg_key", "unsigned long long", const=True)
+            ps.ptr("stg_val", self.val_t, const=True)
+            ps.ptr("stg_ord", "unsigned int")
+            ps.ptr("stg_top", "unsigned long long")
+            ps.ptr("run_tab", "unsigned int", const=True)
+            ps.ptr("run_rows", "int")
+            ps.val("rows_cap", "int")
+            ps.val("n_tiles", "int")
+            ps.val("tile_cap", "long long")
         if self.reset_mode == "deferred":
             ps.ptr("spiked", "unsigned char")
         ps.ptr("status", "int")
@@ -202,7 +373,7 @@ class NeuronKernel:
         self.source = self._source(ps)
 
     def _source(self, ps):
-        T = T_TILE
+        T = self.T
         nw = T // 64
         k = [source_header(T), ps.c_decl()]
         kname = f"{'flush' if self.flush else 'step'}_{self.name}"
@@ -217,69 +388,23 @@ class NeuronKernel:
         k.append("    const bool hb_active = i < N;")
         k.append("    bool hb_cond = false;")
         k.append("    (void)t; (void)dt; (void)_idx;")
-        # ---- fold: this tile's events, counting-sorted by target, each run sorted by key.
-        # Runs live in LDS when the tile received at most LDS_CAP events, otherwise as
-        # an index permutation in global memory (bursts).
+        # state of this thread's neuron in registers (the fold updates them)
+        for v, ct in self.arrays.items():
+            if ct == "bool":
+                k.append(f"    bool {v} = hb_active ? p.a_{v}[i] != 0 : false;")
+            else:
+                k.append(f"    {ct} {v} = hb_active ? p.a_{v}[i] : ({ct})0;")
         if self.fold:
-            cap, vt = self.cap, self.val_t
-            k += [
-                "    // ---- events delivered by the previous step ----",
-                f"    __shared__ unsigned long long s_key[{cap}];",
-                f"    __shared__ {vt} s_val[{cap}];",
-                f"    __shared__ int s_pos[{T}];",
-                f"    __shared__ int s_sh[{nw + 1}];",
-                "    __shared__ int s_n;",
-                "    if (threadIdx.x == 0) { const int c = p.bkt_cnt[blockIdx.x]; s_n = c < p.bkt_cap ? c : p.bkt_cap; }",
-                "    s_pos[threadIdx.x] = 0;",
-                "    __syncthreads();",
-                "    const int hb_n = s_n;",
-                f"    const bool hb_big = hb_n > {cap};",
-                "    const size_t hb_off = (size_t)blockIdx.x * (size_t)p.bkt_cap;",
-                "    const unsigned long long* hb_gk = p.bkt_key + hb_off;",
-                f"    const {vt}* hb_gv = p.bkt_val + hb_off;",
-                "    int* hb_go = p.bkt_ord + hb_off;",
-                f"    for (int e = threadIdx.x; e < hb_n; e += {T}) atomicAdd(&s_pos[(int)(hb_gk[e] >> {KEY_TGT_SHIFT})], 1);",
-                "    __syncthreads();",
-                "    const int hb_cnt = s_pos[threadIdx.x];",
-                "    int hb_tot;",
-                "    const int hb_beg = hb_block_exscan(hb_cnt, s_sh, &hb_tot);",
-                "    s_pos[threadIdx.x] = hb_beg;",
-                "    __syncthreads();",
-                "    if (!hb_big) {",
-                f"        for (int e = threadIdx.x; e < hb_n; e += {T}) {{",
-                "            const unsigned long long kk = hb_gk[e];",
-                f"            const int q = atomicAdd(&s_pos[(int)(kk >> {KEY_TGT_SHIFT})], 1);",
-                "            s_key[q] = kk;",
-                "            s_val[q] = hb_gv[e];",
-                "        }",
-                "    } else {",
-                f"        for (int e = threadIdx.x; e < hb_n; e += {T}) {{",
-                f"            const int q = atomicAdd(&s_pos[(int)(hb_gk[e] >> {KEY_TGT_SHIFT})], 1);",
-                "            hb_go[q] = e;",
-                "        }",
-                "    }",
-                "    __syncthreads();",
-                "    if (threadIdx.x == 0) p.bkt_cnt[blockIdx.x] = 0;",
-                "    if (!hb_big) hb_sort_kv(s_key + hb_beg, s_val + hb_beg, hb_cnt);",
-                "    else hb_sort_idx(hb_go + hb_beg, hb_gk, hb_cnt);",
-            ]
+            fk, U, W = _fold_source(T, self.cap, self.val_t, self.flush)
+            k += fk
+            apply = ["switch ((int)((hb_k >> 44) & 15)) {"]
+            for rank, var, op in self.fold:
+                apply.append(f"case {rank}: {var} {op} hb_v; break;")
+            apply += ["default: break;", "}"]
+            k += _fold_passes(T, U, W, self.cap, self.val_t, apply, apply)
         # ---- per-neuron body ------------------------------------------------------
         k.append("    if (hb_active) {")
         body = []
-        for v, ct in self.arrays.items():
-            body.append(f"bool {v} = p.a_{v}[i] != 0;" if ct == "bool" else f"{ct} {v} = p.a_{v}[i];")
-        if self.fold:
-            body.append("for (int e = hb_beg; e < hb_beg + hb_cnt; e++) {")
-            body.append("    unsigned long long hb_k;")
-            body.append(f"    {self.val_t} hb_v;")
-            body.append("    if (!hb_big) { hb_k = s_key[e]; hb_v = s_val[e]; }")
-            body.append("    else { const int r = hb_go[e]; hb_k = hb_gk[r]; hb_v = hb_gv[r]; }")
-            body.append("    switch ((int)((hb_k >> 44) & 15)) {")
-            for rank, var, op in self.fold:
-                body.append(f"    case {rank}: {var} {op} hb_v; break;")
-            body.append("    default: break;")
-            body.append("    }")
-            body.append("}")
         if self.reset_mode == "deferred":
             body.append("if (p.spiked[i]) { // reset of the previous step (after its propagation)")
             body += ["    " + x for x in self._block("rs", self.rs, {})]
@@ -384,14 +509,25 @@ class NeuronKernel:
 
 # ---------------------------------------------------------------------------
 class GatherKernel:
-    """Per-pathway event gather: spikes of step-d -> CSR segments of delay d -> buckets."""
+    """Per-pathway event gather: spikes of step-d -> CSR segments of delay d ->
+    tile-sorted runs in the target group's staging area.
 
-    def __init__(self, name, rank, n_seg, dmin, on_pre, syn_arrays, scalars, val_t, bucket_cap, float_t,
-                 target_var, count_synapses=False):
+    Work is split in units of `pairs` (delay, spike) pairs (one per thread of the
+    first `pairs` threads).  Their segments are flattened into events, handled in
+    windows of GATHER_BLOCK * GATHER_EVENTS_PER_THREAD events held in registers:
+    counted per target tile in LDS, written as one run (two global atomics per
+    window: staging space and run-table row)."""
+
+    def __init__(self, name, rank, n_seg, dmin, on_pre, syn_arrays, scalars, val_t, float_t, target_var,
+                 n_tiles, tile, count_synapses=False, pairs=256):
         if rank >= MAX_RANK:
             raise NotImplementedError(f"more than {MAX_RANK} pathways into one group")
         if n_seg > MAX_SEG:
             raise NotImplementedError(f"delay range of more than {MAX_SEG} steps")
+        if n_tiles > 16384:
+            raise NotImplementedError("more than 16384 target tiles")
+        if pairs & (pairs - 1) or not 32 <= pairs <= GATHER_BLOCK:
+            raise ValueError("pairs per unit must be a power of two in [32, GATHER_BLOCK]")
         self.name = name
         self.rank = rank
         self.n_seg = n_seg
@@ -400,9 +536,11 @@ class GatherKernel:
         self.syn_arrays = syn_arrays  # {name: ctype}
         self.scalars = scalars        # {name: (value, ctype)}
         self.val_t = val_t
-        self.cap = bucket_cap
         self.float_t = float_t
         self.target_var = target_var
+        self.n_tiles = n_tiles
+        self.tile = tile
+        self.pairs = pairs
         self.count_synapses = count_synapses
         self.kernel_name = f"gather_{name}"
         self._build()
@@ -417,120 +555,191 @@ class GatherKernel:
         ps.val("ring_slots", "int")
         ps.val("src0", "int")
         ps.val("src1", "int")
-        ps.val("pad0", "int")
+        ps.val("rows_cap", "int")
         ps.ptr("seg_ptr", "unsigned int", const=True)
         ps.ptr("post", "int", const=True)
         for v, ct in self.syn_arrays.items():
             ps.ptr(f"s_{v}", ct, const=True)
         for n, (_, ct) in sorted(self.scalars.items()):
             ps.val(f"c_{n}", ct)
-        ps.ptr("bkt_cnt", "int")
-        ps.ptr("bkt_key", "unsigned long long")
-        ps.ptr("bkt_val", self.val_t)
-        ps.val("bkt_cap", "int")
-        ps.val("pad_bkt", "int")
+        ps.ptr("stg_key", "unsigned long long")
+        ps.ptr("stg_val", self.val_t)
+        ps.ptr("stg_top", "unsigned long long")
+        ps.val("stg_cap", "unsigned long long")
+        ps.ptr("run_tab", "unsigned int")
+        ps.ptr("run_rows", "int")
         ps.ptr("status", "int")
         ps.ptr("events", "unsigned long long")  # per-workgroup delivered-event totals
         if self.count_synapses:
             ps.ptr("syn_count", "int")           # per-synapse event counts (CSR order)
         self.struct = ps
-        T = T_TILE
+        GT = GATHER_BLOCK
+        U = GATHER_EVENTS_PER_THREAD
+        WN = GT * U
+        PC = self.pairs
         NS = self.n_seg
-        cap = self.cap
-        # value expression: statements before the target write are temporaries
+        NT = self.n_tiles
+        TT = self.tile
+        VT = self.val_t
+        # value of the on_pre statement (temporaries first)
         types = dict(self.syn_arrays)
         types["_postsynaptic_idx"] = "int"
         types["_idx"] = "int"
         for n, (_, ct) in self.scalars.items():
             types[n] = ct
         names = {n: f"p.c_{n}" for n in self.scalars}
-        lines = []
-        declared = set()
-        from .codegen import translate_block as tb
-
-        body = self.on_pre[:-1]
-        lines += tb(body, {}, types, declared, self.float_t)
-        R = Renderer(types)
-        c, _ = R.render(self.on_pre[-1]["expr"])
-        lines.append(f"const {self.val_t} hb_val = ({self.val_t})({c});")
-        import re
-
-        if names:
-            pat = re.compile(r"(?<![\w.])(" + "|".join(re.escape(k) for k in sorted(names, key=len, reverse=True)) + r")(?!\w)")
-            lines = [pat.sub(lambda m: names[m.group(1)], ln) for ln in lines]
-        loads = [f"const {ct} {v} = p.s_{v}[e];" for v, ct in self.syn_arrays.items()]
-        k = [source_header(T), ps.c_decl()]
-        k.append(f'extern "C" __global__ void __launch_bounds__({T}) {self.kernel_name}(const {ps.name} p) {{')
-        k.append("    const long long step = p.clock[0] + p.step_off;")
-        k.append(f"    __shared__ int s_cum[{NS + 1}];")
-        k.append(f"    for (int s = threadIdx.x; s < {NS}; s += {T}) {{")
-        k.append(f"        const long long ps_ = step - {self.dmin} - s;")
-        k.append("        s_cum[s + 1] = ps_ >= 0 ? p.ring_cnt[(int)(ps_ % p.ring_slots)] : 0;")
-        k.append("    }")
-        k.append("    if (threadIdx.x == 0) s_cum[0] = 0;")
-        k.append("    __syncthreads();")
-        k.append(f"    if (threadIdx.x == 0) for (int s = 1; s <= {NS}; s++) s_cum[s] += s_cum[s - 1];")
-        k.append("    __syncthreads();")
-        k.append(f"    const int total = s_cum[{NS}];")
-        k.append("    __shared__ unsigned long long s_ev;")
-        k.append("    if (threadIdx.x == 0) s_ev = 0ull;")
-        k.append("    __syncthreads();")
-        k.append("    unsigned long long hb_ev = 0ull;")
-        k.append("    const int lane = threadIdx.x & 63;")
-        k.append("    const int nwaves = (gridDim.x * blockDim.x) >> 6;")
-        k.append("    for (int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < total; q += nwaves) {")
-        k.append(f"        int lo = 0, hi = {NS};")
-        k.append("        while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (s_cum[m] <= q) lo = m; else hi = m; }")
-        k.append("        const int seg = lo;")
-        k.append(f"        const long long sstep = step - {self.dmin} - seg;")
-        k.append("        const int slot = (int)(sstep % p.ring_slots);")
-        k.append("        const int src = p.ring_spk[(size_t)slot * p.ring_stride + (q - s_cum[seg])];")
-        k.append("        if (src < p.src0 || src >= p.src1) continue;")
-        k.append(f"        const size_t row = (size_t)(src - p.src0) * {NS} + seg;")
-        k.append("        const unsigned int a = p.seg_ptr[row], b = p.seg_ptr[row + 1];")
-        k.append("        hb_ev += (unsigned long long)(b - a);")
-        k.append(f"        const unsigned long long khi = ((unsigned long long){self.rank} << 44) | "
-                 f"((unsigned long long)({NS - 1} - seg) << 32);")
-        k.append("        for (unsigned int e = a + lane; e < b; e += 64) {")
-        k.append("            const int _postsynaptic_idx = p.post[e];")
-        k.append("            const int _idx = (int)e;")
-        k.extend("            " + x for x in loads)
-        k.extend("            " + x for x in lines)
-        k.append(f"            const int tile = _postsynaptic_idx / {T};")
-        k.append(f"            const unsigned long long key = ((unsigned long long)(_postsynaptic_idx - tile * {T}) << {KEY_TGT_SHIFT}) | khi | (unsigned long long)e;")
+        lines = translate_block(self.on_pre[:-1], {}, types, set(), self.float_t)
+        c, _ = Renderer(types).render(self.on_pre[-1]["expr"])
+        lines.append(f"hb_gv[hb_u] = ({VT})({c});")
+        lines = _rename(lines, names)
+        k = [source_header(GT), ps.c_decl()]
+        k.append(f'extern "C" __global__ void __launch_bounds__({GT}) {self.kernel_name}(const {ps.name} p) {{')
+        k += [
+            "    const long long step = p.clock[0] + p.step_off;",
+            "    const int par = (int)(step & 1);",
+            f"    __shared__ int s_cum[{NS + 1}];",
+            f"    __shared__ unsigned int s_a[{PC}];",
+            f"    __shared__ int s_o[{PC}];",
+            f"    __shared__ unsigned short s_sg[{PC}];",
+            f"    __shared__ int s_hist[{NT}];",
+            f"    __shared__ int s_sh[{GT // 64 + 1}];",
+            "    __shared__ unsigned long long s_base;",
+            "    __shared__ int s_row;",
+            f"    for (int s = threadIdx.x; s < {NS}; s += {GT}) {{",
+            f"        const long long ps_ = step - {self.dmin} - s;",
+            "        s_cum[s + 1] = ps_ >= 0 ? p.ring_cnt[(int)(ps_ % p.ring_slots)] : 0;",
+            "    }",
+            "    if (threadIdx.x == 0) s_cum[0] = 0;",
+            f"    for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
+            "    __syncthreads();",
+            f"    if (threadIdx.x == 0) for (int s = 1; s <= {NS}; s++) s_cum[s] += s_cum[s - 1];",
+            "    __syncthreads();",
+            f"    const int P = s_cum[{NS}];",
+            f"    const int n_units = (P + {PC - 1}) / {PC};",
+            "    unsigned long long hb_ev = 0ull;",
+            "    for (int c = blockIdx.x; c < n_units; c += gridDim.x) {",
+            f"        const int q0 = c * {PC};",
+            f"        const int nq = P - q0 < {PC} ? P - q0 : {PC};",
+            "        // one (delay, spike) pair per thread: its synapse segment",
+            "        unsigned int a = 0, len = 0;",
+            "        int seg = 0;",
+            "        if ((int)threadIdx.x < nq) {",
+            "            const int qa = q0 + (int)threadIdx.x;",
+            f"            int lo = 0, hi = {NS};",
+            "            while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (s_cum[m] <= qa) lo = m; else hi = m; }",
+            "            seg = lo;",
+            f"            const long long sstep = step - {self.dmin} - seg;",
+            "            const int slot = (int)(sstep % p.ring_slots);",
+            "            const int src = p.ring_spk[(size_t)slot * p.ring_stride + (qa - s_cum[seg])];",
+            "            if (src >= p.src0 && src < p.src1) {",
+            f"                const size_t row = (size_t)(src - p.src0) * {NS} + seg;",
+            "                a = p.seg_ptr[row];",
+            "                len = p.seg_ptr[row + 1] - a;",
+            "            }",
+            "        }",
+            "        int E;",
+            "        const int ex = hb_block_exscan((int)len, s_sh, &E);",
+            f"        if ((int)threadIdx.x < {PC}) {{ s_a[threadIdx.x] = a; s_o[threadIdx.x] = ex; s_sg[threadIdx.x] = (unsigned short)seg; }}",
+            "        __syncthreads();",
+            f"        for (int w0 = 0; w0 < E; w0 += {WN}) {{",
+            f"            const int Ew = E - w0 < {WN} ? E - w0 : {WN};",
+            f"            int hb_gt[{U}];",
+            f"            unsigned long long hb_gk[{U}];",
+            f"            {VT} hb_gv[{U}];",
+            "            // the window's events in registers, counted per target tile: positions first,",
+            "            // then all loads in flight at once, then keys and values",
+            f"            unsigned int hb_ge[{U}];",
+            f"            unsigned short hb_gs[{U}];",
+            f"            int hb_gp[{U}];",
+        ]
+        k += [f"            {ct} hb_sv_{v}[{U}];" for v, ct in self.syn_arrays.items()]
+        k += [
+            "            #pragma unroll",
+            f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+            f"                const int ev = w0 + hb_u * {GT} + (int)threadIdx.x;",
+            "                const int evc = ev < E ? ev : E - 1;",
+            "                int lo = 0, hi = nq;",
+            "                while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (s_o[m] <= evc) lo = m; else hi = m; }",
+            "                hb_ge[hb_u] = s_a[lo] + (unsigned int)(evc - s_o[lo]);",
+            "                hb_gs[hb_u] = s_sg[lo];",
+            "                hb_gt[hb_u] = ev < E ? 0 : -1;",
+            "            }",
+            "            #pragma unroll",
+            f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+            "                hb_gp[hb_u] = p.post[hb_ge[hb_u]];",
+        ]
+        k += [f"                hb_sv_{v}[hb_u] = p.s_{v}[hb_ge[hb_u]];" for v in self.syn_arrays]
+        k += [
+            "            }",
+            "            #pragma unroll",
+            f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+            "                if (hb_gt[hb_u] >= 0) {",
+            "                    const unsigned int e = hb_ge[hb_u];",
+            "                    const int _postsynaptic_idx = hb_gp[hb_u];",
+            "                    const int _idx = (int)e;",
+            "                    (void)_idx;",
+        ]
+        k += [f"                    const {ct} {v} = hb_sv_{v}[hb_u];" for v, ct in self.syn_arrays.items()]
+        k += ["                    " + x for x in lines]
+        k += [
+            f"                    const int tile = _postsynaptic_idx / {TT};",
+            "                    hb_gt[hb_u] = tile;",
+            f"                    hb_gk[hb_u] = ((unsigned long long)(_postsynaptic_idx - tile * {TT}) << {KEY_TGT_SHIFT})",
+            f"                        | ((unsigned long long){self.rank} << 44) | ((unsigned long long)({NS - 1} - hb_gs[hb_u]) << 32)",
+            "                        | (unsigned long long)e;",
+            "                }",
+            "            }",
+            "            #pragma unroll",
+            f"            for (int hb_u = 0; hb_u < {U}; hb_u++) if (hb_gt[hb_u] >= 0) atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
+            "            if (threadIdx.x == 0) {",
+            "                s_base = atomicAdd(p.stg_top + par, (unsigned long long)Ew);",
+            "                s_row = atomicAdd(p.run_rows + par, 1);",
+            "            }",
+            "            __syncthreads();",
+            "            const unsigned long long base = s_base;",
+            "            const int row = s_row;",
+            "            const bool ok = base + (unsigned long long)Ew <= p.stg_cap && row < p.rows_cap;",
+            "            if (threadIdx.x == 0 && !ok) atomicOr(p.status, 1);",
+            f"            unsigned int* tab = p.run_tab + (size_t)row * {NT + 1};",
+            "            // run-table row: absolute start of every tile's events (all empty if they do not fit)",
+            "            int carry = 0;",
+            f"            for (int t0 = 0; t0 < {NT}; t0 += {GT}) {{",
+            "                const int t = t0 + (int)threadIdx.x;",
+            f"                const int v = t < {NT} ? s_hist[t] : 0;",
+            "                int tot;",
+            "                const int x = hb_block_exscan(v, s_sh, &tot) + carry;",
+            f"                if (t < {NT}) {{",
+            "                    s_hist[t] = x;",
+            "                    if (row < p.rows_cap) tab[t] = ok ? (unsigned int)(base + (unsigned long long)x) : 0u;",
+            "                }",
+            "                carry += tot;",
+            "            }",
+            f"            if (threadIdx.x == 0 && row < p.rows_cap) tab[{NT}] = ok ? (unsigned int)(base + (unsigned long long)Ew) : 0u;",
+            "            __syncthreads();",
+            "            if (ok) {",
+            "                #pragma unroll",
+            f"                for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+            "                    if (hb_gt[hb_u] >= 0) {",
+            "                        const unsigned long long dst = base + (unsigned long long)atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
+            "                        p.stg_key[dst] = hb_gk[hb_u];",
+            "                        p.stg_val[dst] = hb_gv[hb_u];",
+        ]
         if self.count_synapses:
-            k.append("            p.syn_count[e] += 1;")
-        k.append("            const int pos = atomicAdd(&p.bkt_cnt[tile], 1);")
-        k.append("            if (pos < p.bkt_cap) {")
-        k.append("                p.bkt_key[(size_t)tile * (size_t)p.bkt_cap + pos] = key;")
-        k.append("                p.bkt_val[(size_t)tile * (size_t)p.bkt_cap + pos] = hb_val;")
-        k.append("            } else {")
-        k.append("                atomicOr(p.status, 1);")
-        k.append("            }")
-        k.append("        }")
-        k.append("    }")
-        k.append("    if (lane == 0 && hb_ev) atomicAdd(&s_ev, hb_ev);")
-        k.append("    __syncthreads();")
-        k.append("    if (threadIdx.x == 0) p.events[blockIdx.x] += s_ev;")
-        k.append("}")
+            k.append("                        p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")
+        k += [
+            "                    }",
+            "                }",
+            "            }",
+            "            __syncthreads();",
+            f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
+            "            __syncthreads();",
+            "        }",
+            "        if (threadIdx.x == 0) hb_ev += (unsigned long long)E;",
+            "    }",
+            "    if (threadIdx.x == 0) p.events[blockIdx.x] += hb_ev;",
+            "}",
+        ]
         self.source = "\n".join(k) + "\n"
 
 
-def struct_offset(struct_cls, field):
-    return getattr(struct_cls, field).offset
-
-
-def fill_struct(struct_cls, values):
-    s = struct_cls()
-    for name, _ in struct_cls._fields_:
-        if name in values:
-            v = values[name]
-            setattr(s, name, v)
-    return s
-
-
-def struct_bytes(s):
-    return ctypes.string_at(ctypes.addressof(s), ctypes.sizeof(s))
-
-
-__all__ = ["CodeSpec", "NeuronKernel", "GatherKernel", "scalar_values", "T_TILE", "c_literal"]
+__all__ = ["CodeSpec", "NeuronKernel", "GatherKernel", "scalar_values", "identifiers", "c_literal"]

@@ -20,10 +20,12 @@ import numpy as np
 from . import _lib
 from .codegen import ctype_of
 from .kernels import (
-    T_TILE,
+    GATHER_BLOCK,
+    GATHER_EVENTS_PER_THREAD,
     CodeSpec,
     GatherKernel,
     NeuronKernel,
+    fold_capacity,
     identifiers,
     scalar_values,
 )
@@ -467,14 +469,21 @@ class StateMonitor:
 class Network:
     """Runs the step loop on the device (Network.run, brian2/core/network.py:1047-1292)."""
 
-    def __init__(self, *objs, dt=1e-4, bucket_cap=2048, graph_steps=100, bucket_mem_frac=0.3):
-        """bucket_cap: events per tile folded in LDS (larger bursts use global memory);
-        bucket_mem_frac: share of free device memory the per-tile event buckets may use."""
+    def __init__(self, *objs, dt=1e-4, bucket_cap=None, graph_steps=100, bucket_mem_frac=0.3, tile=None,
+                 gather_pairs=None):
+        """bucket_cap: events of a tile sorted in LDS at a time (default: what fits two
+        workgroups per CU); bucket_mem_frac: share of free device memory the event
+        staging may use; tile: neurons per workgroup of the step kernels (default: 1024
+        for groups of at least 65536 neurons, else 256); gather_pairs: (delay, spike)
+        pairs per gather unit (default: about one register window of events).  These
+        shape the launches only: results do not depend on them."""
         self.objs = []
         self.dt = float(dt)
-        self.bucket_cap = int(bucket_cap)
+        self.bucket_cap = None if bucket_cap is None else int(bucket_cap)
         self.bucket_mem_frac = float(bucket_mem_frac)
         self.graph_steps = int(graph_steps)
+        self.tile = tile
+        self.gather_pairs = gather_pairs
         self.t_step = 0
         self._built = False
         for o in objs:
@@ -511,6 +520,7 @@ class Network:
             s._build_csr(self.dt)
         # per group: device state, spike ring
         for g in groups:
+            g.tile = int(self.tile or (1024 if g.N >= 65536 else 256))
             dmax = max([s.csr["dmin"] + s.csr["n_seg"] - 1 for s in paths if s.source is g] + [0])
             g.ring_slots = dmax + 2
             g.ring_spk = zeros(g.ring_slots * g.N, "int")
@@ -531,24 +541,7 @@ class Network:
             val_t = "double" if "double" in vts else ("long long" if "long long" in vts else
                                                         ("int" if "int" in vts else "float"))
             g.val_t = val_t
-            g.n_tiles = (g.N + T_TILE - 1) // T_TILE
-            # Each synapse delivers at most one event per step, so a tile can never
-            # receive more events in one step than the in-degree of its targets: size
-            # every tile's bucket by the largest such in-degree (within a memory budget).
-            indeg = torch.zeros(g.n_tiles, dtype=torch.int64, device="cuda")
-            for s in incoming:
-                post = s.csr["post"][: s.csr["n_syn"]]
-                indeg += torch.bincount(torch.div(post, T_TILE, rounding_mode="floor").long(), minlength=g.n_tiles)
-            need = max(1, int(indeg.max().item()))
-            rec_bytes = 8 + (8 if val_t in ("double", "long long") else 4) + 4
-            free, _ = torch.cuda.mem_get_info()
-            budget = int(free * self.bucket_mem_frac) // (rec_bytes * g.n_tiles)
-            g.bkt_capacity = int(min(need, max(budget, 1), 2**31 - 1))
-            g.lds_cap = int(min(self.bucket_cap, g.bkt_capacity))
-            g.bkt_cnt = zeros(g.n_tiles, "int")
-            g.bkt_key = torch.zeros(g.n_tiles * g.bkt_capacity, dtype=torch.int64, device="cuda")
-            g.bkt_val = zeros(g.n_tiles * g.bkt_capacity, val_t)
-            g.bkt_ord = zeros(g.n_tiles * g.bkt_capacity, "int")
+            self._alloc_staging(g, incoming)
         self._plan = ctypes.c_void_p()
         _lib.check(L.hb_plan_create(D.ctx, ptr(self.clock), ctypes.byref(self._plan)), "hb_plan_create")
         self._launches = []
@@ -560,6 +553,58 @@ class Network:
         for s in paths:
             self._build_gather(s)
         self._built = True
+
+    def _alloc_staging(self, g, incoming):
+        """Event staging of one target group.
+
+        stg_key/stg_val: the events of one step, as tile-sorted runs (one per gather
+        window).  Each synapse delivers at most one event per step, so the number of
+        incoming synapses bounds a step's events; run_tab: one row of n_tiles + 1
+        offsets per window, bounded by the units and windows of a step in which every
+        source spikes; stg_ord: per-tile index lists for targets with more events in
+        one step than half the fold's LDS capacity, bounded by the tile's in-degree.
+        Each bound is honoured unless it exceeds the memory budget, in which case an
+        overflow is reported (status bits 1, 4)."""
+        torch = _torch()
+        T = g.tile
+        g.n_tiles = (g.N + T - 1) // T
+        vbytes = 8 if g.val_t in ("double", "long long") else 4
+        g.lds_cap = int(self.bucket_cap or fold_capacity(T, g.val_t))
+        window = GATHER_BLOCK * GATHER_EVENTS_PER_THREAD
+        total = sum(int(s.csr["n_syn"]) for s in incoming)
+        rows = 0
+        for s in incoming:
+            nsrc = s.src_range[1] - s.src_range[0]
+            pairs_max = nsrc * s.csr["n_seg"]
+            # pairs per unit: about one window of events per unit on average
+            avg = s.csr["n_syn"] / max(1, pairs_max)
+            pc = 32
+            while pc < GATHER_BLOCK and (2 * pc) * avg <= 0.8 * window:
+                pc *= 2
+            s.pairs = int(self.gather_pairs or pc)
+            s.units_max = -(-pairs_max // s.pairs)
+            rows += s.units_max + -(-int(s.csr["n_syn"]) // window)
+        indeg_t = torch.zeros(g.n_tiles, dtype=torch.int64, device="cuda")
+        indeg_n = torch.zeros(g.N, dtype=torch.int64, device="cuda")
+        for s in incoming:
+            post = s.csr["post"][: s.csr["n_syn"]].long()
+            indeg_n += torch.bincount(post, minlength=g.N)
+        indeg_t += torch.nn.functional.pad(indeg_n, (0, g.n_tiles * T - g.N)).view(g.n_tiles, T).sum(1)
+        need_tile = int(indeg_t.max().item()) if total else 0
+        need_target = int(indeg_n.max().item()) if total else 0
+        del indeg_t, indeg_n
+        free, _ = torch.cuda.mem_get_info()
+        budget = int(free * self.bucket_mem_frac)
+        g.stg_cap = int(max(1, min(total, 2**32 - 1, (budget // 2) // (8 + vbytes))))
+        g.rows_cap = int(max(1, min(rows, 2**31 - 1, (budget // 4) // (4 * (g.n_tiles + 1)))))
+        big = need_target > g.lds_cap // 2
+        g.tile_cap = int(0 if not big else min(need_tile, (budget // 4) // (4 * g.n_tiles)))
+        g.stg_key = torch.empty(g.stg_cap, dtype=torch.int64, device="cuda")
+        g.stg_val = torch.empty(g.stg_cap, dtype=torch_dtype(g.val_t), device="cuda")
+        g.run_tab = torch.zeros(g.rows_cap * (g.n_tiles + 1), dtype=torch.int32, device="cuda")
+        g.stg_ord = torch.empty(max(1, g.n_tiles * g.tile_cap), dtype=torch.int32, device="cuda")
+        g.stg_top = torch.zeros(2, dtype=torch.int64, device="cuda")
+        g.run_rows = torch.zeros(2, dtype=torch.int32, device="cuda")
 
     def _value_type(self, s):
         from .codegen import Renderer
@@ -628,7 +673,7 @@ class Network:
                          spikemon=g.smon.range if g.smon else None,
                          statemon=g.stmon.variables if g.stmon else None,
                          uses_refractory=g.uses_refractory,
-                         statemon_when=g.stmon.when if g.stmon else "start")
+                         statemon_when=g.stmon.when if g.stmon else "start", tile=g.tile)
         g.kernel = K
         g.fn = D.function(K.source, K.kernel_name)
         g.struct_cls = K.struct.ctypes_struct()
@@ -646,13 +691,13 @@ class Network:
             m.rec_buf = None
         g.launch_index = len(self._launches)
         args = self._group_args(g)
-        self._add_launch(g.fn, (g.N + T_TILE - 1) // T_TILE, T_TILE, args, g.struct_cls)
+        self._add_launch(g.fn, (g.N + g.tile - 1) // g.tile, g.tile, args, g.struct_cls)
         end_mon = g.stmon is not None and g.stmon.when == "end"
         if g.incoming or mode == "deferred" or end_mon:
             F = NeuronKernel(g.name, g.N, g.vars, g.float_t, g.su, g.th, g.rs, mode, scalars, fold,
                              getattr(g, "lds_cap", 0), getattr(g, "val_t", "double"), flush=True,
                              uses_refractory=g.uses_refractory,
-                             statemon=g.stmon.variables if end_mon else None, statemon_when="end")
+                             statemon=g.stmon.variables if end_mon else None, statemon_when="end", tile=g.tile)
             g.flush_fn = D.function(F.source, F.kernel_name)
             g.flush_struct_cls = F.struct.ctypes_struct()
         else:
@@ -668,8 +713,9 @@ class Network:
         for (tg, n), (val, ct) in g.kernel.scalars.items():
             v[f"{tg}_{n}"] = val
         if g.incoming:
-            v.update(bkt_cnt=g.bkt_cnt.data_ptr(), bkt_key=g.bkt_key.data_ptr(), bkt_val=g.bkt_val.data_ptr(),
-                     bkt_ord=g.bkt_ord.data_ptr(), bkt_cap=g.bkt_capacity)
+            v.update(stg_key=g.stg_key.data_ptr(), stg_val=g.stg_val.data_ptr(), stg_ord=g.stg_ord.data_ptr(),
+                     stg_top=g.stg_top.data_ptr(), run_tab=g.run_tab.data_ptr(), run_rows=g.run_rows.data_ptr(),
+                     rows_cap=g.rows_cap, n_tiles=g.n_tiles, tile_cap=g.tile_cap)
         if getattr(g, "spiked", None) is not None:
             v["spiked"] = g.spiked.data_ptr()
         if not flush and g.smon:
@@ -721,7 +767,8 @@ class Network:
         sc = scalar_values(s.spec, host, g.float_t)
         types_syn = s.syn_ctypes()
         K = GatherKernel(s.name, rank, s.csr["n_seg"], s.csr["dmin"], s.spec.vector, types_syn, sc, g.val_t,
-                         self.bucket_cap, g.float_t, s.target_var, count_synapses=s.count_events)
+                         g.float_t, s.target_var, g.n_tiles, g.tile, count_synapses=s.count_events,
+                         pairs=s.pairs)
         s.kernel = K
         s.fn = D.function(K.source, K.kernel_name)
         cls = K.struct.ctypes_struct()
@@ -729,14 +776,15 @@ class Network:
         v = {"clock": self.clock.data_ptr(), "step_off": 0, "ring_spk": s.source.ring_spk.data_ptr(),
              "ring_cnt": s.source.ring_cnt.data_ptr(), "ring_stride": s.source.N, "ring_slots": s.source.ring_slots,
              "src0": s.src_range[0], "src1": s.src_range[1], "seg_ptr": s.csr["seg_ptr"].data_ptr(),
-             "post": s.csr["post"].data_ptr(), "bkt_cnt": g.bkt_cnt.data_ptr(), "bkt_key": g.bkt_key.data_ptr(),
-             "bkt_val": g.bkt_val.data_ptr(), "bkt_cap": g.bkt_capacity, "status": self.status.data_ptr()}
+             "post": s.csr["post"].data_ptr(), "rows_cap": g.rows_cap, "stg_key": g.stg_key.data_ptr(),
+             "stg_val": g.stg_val.data_ptr(), "stg_top": g.stg_top.data_ptr(), "stg_cap": g.stg_cap,
+             "run_tab": g.run_tab.data_ptr(), "run_rows": g.run_rows.data_ptr(), "status": self.status.data_ptr()}
         for n in s.syn_vars:
             v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
         for n, (val, ct) in sc.items():
             v[f"c_{n}"] = val
         nsrc = s.src_range[1] - s.src_range[0]
-        grid = int(min(2048, max(4, (nsrc * s.csr["n_seg"]) // 256)))
+        grid = int(min(256, max(1, s.units_max)))
         s.events_dev = _torch().zeros(grid, dtype=_torch().int64, device="cuda")
         v["events"] = s.events_dev.data_ptr()
         if s.count_events:
@@ -747,7 +795,8 @@ class Network:
             if name in v:
                 setattr(a, name, v[name])
         s.launch_index = len(self._launches)
-        self._add_launch(s.fn, grid, 256, a, cls)
+        s.args, s.grid = a, grid
+        self._add_launch(s.fn, grid, GATHER_BLOCK, a, cls)
 
     def _add_launch(self, fn, grid, block, args, cls, step_field="step_off"):
         off = getattr(cls, step_field).offset
@@ -800,8 +849,10 @@ class Network:
                 off = type(a).step_off.offset
                 a.step_off = 0
                 buf = bytes(a)
-                grid = (g.N + T_TILE - 1) // T_TILE
-                _lib.check(L.hb_launch(g.flush_fn, grid, T_TILE, 0, stream, buf, len(buf)), "flush")
+                grid = (g.N + g.tile - 1) // g.tile
+                _lib.check(L.hb_launch(g.flush_fn, grid, g.tile, 0, stream, buf, len(buf)), "flush")
+                if getattr(g, "incoming", None):
+                    g.run_rows.zero_()  # the flushed events are not folded again by the next run
         for g in self.groups:
             m = getattr(g, "stmon", None)
             if m is not None and n_steps > 0:
@@ -832,8 +883,11 @@ class Network:
     def _status_message(self, st):
         msg = []
         if st & 1:
-            msg.append(f"event bucket overflow (a tile of {T_TILE} neurons received more synaptic events in one step "
-                       f"than its bucket holds); build the Network with a larger bucket_mem_frac")
+            msg.append("event staging overflow (more synaptic events or gather chunks in one step than the "
+                       "staging area holds); build the Network with a larger bucket_mem_frac")
+        if st & 4:
+            msg.append("event burst overflow (a tile received more synaptic events in one step than its index "
+                       "list holds); build the Network with a larger bucket_mem_frac")
         if st & 2:
             msg.append("spike monitor buffer overflow")
         return "; ".join(msg) or f"device status {st}"

@@ -142,7 +142,7 @@ def cuba_v0(seed, N):
 
 
 def cuba_network(N=4000, NE=None, p=0.02, seed=SEED, float_t="double", we=None, wi=None, monitor=True,
-                 record=None, bucket_cap=2048, graph_steps=100, device_conn=True):
+                 record=None, bucket_cap=None, graph_steps=100, device_conn=True, network_options=None):
     """CUBA (examples/CUBA.py), connectivity Bernoulli(p) over all pairs, delay 0."""
     NE = N * 4 // 5 if NE is None else NE
     m = Model.load("cuba")
@@ -177,12 +177,12 @@ def cuba_network(N=4000, NE=None, p=0.02, seed=SEED, float_t="double", we=None, 
     if record is not None:
         st = StateMonitor(G, ["v", "ge", "gi"], record=record)
         objs.append(st)
-    net = Network(*objs, dt=1e-4, bucket_cap=bucket_cap, graph_steps=graph_steps)
+    net = Network(*objs, dt=1e-4, **dict(dict(bucket_cap=bucket_cap, graph_steps=graph_steps), **(network_options or {})))
     return dict(net=net, G=G, Ce=Ce, Ci=Ci, mon=mon, state=st)
 
 
 def cuba_synw_network(N=5000, K=1000, dmin=1, dmax=50, seed=SEED, ginh=0.4, float_t="double", monitor=True,
-                      record=None, bucket_cap=2048, graph_steps=100):
+                      record=None, bucket_cap=None, graph_steps=100, network_options=None):
     """CUBA with fixed in-degree K, per-synapse fp32 weights and delays (the 1M-neuron workload)."""
     NE = N * 4 // 5
     m = Model.load("cuba_synw")
@@ -207,15 +207,15 @@ def cuba_synw_network(N=5000, K=1000, dmin=1, dmax=50, seed=SEED, ginh=0.4, floa
     if record is not None:
         st = StateMonitor(G, ["v", "ge", "gi"], record=record)
         objs.append(st)
-    net = Network(*objs, dt=1e-4, bucket_cap=bucket_cap, graph_steps=graph_steps)
+    net = Network(*objs, dt=1e-4, **dict(dict(bucket_cap=bucket_cap, graph_steps=graph_steps), **(network_options or {})))
     return dict(net=net, G=G, Ce=Ce, Ci=Ci, mon=mon, state=st, n_syn=n_syn, we_base=we_base, wi_base=wi_base)
 
 
 COBAHH = dict(El=-0.06, taue=0.005, taui=0.01)
 
 
-def cobahh_network(N=4000, p=0.02, seed=SEED, float_t="double", monitor=True, record=None, bucket_cap=2048,
-                   graph_steps=100):
+def cobahh_network(N=4000, p=0.02, seed=SEED, float_t="double", monitor=True, record=None, bucket_cap=None,
+                   graph_steps=100, network_options=None):
     """COBAHH (examples/COBAHH.py): Pe = P[:0.8N] -> ge += we, Pi = P[0.8N:] -> gi += wi."""
     NE = N * 4 // 5
     m = Model.load("cobahh")
@@ -237,7 +237,7 @@ def cobahh_network(N=4000, p=0.02, seed=SEED, float_t="double", monitor=True, re
     if record is not None:
         st = StateMonitor(G, ["v", "m", "n", "h", "ge", "gi"], record=record)
         objs.append(st)
-    net = Network(*objs, dt=1e-4, bucket_cap=bucket_cap, graph_steps=graph_steps)
+    net = Network(*objs, dt=1e-4, **dict(dict(bucket_cap=bucket_cap, graph_steps=graph_steps), **(network_options or {})))
     return dict(net=net, G=G, Ce=Ce, Ci=Ci, mon=mon, state=st)
 
 

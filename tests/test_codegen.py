@@ -116,19 +116,30 @@ def _no_float_atomics(source):
         assert "val" not in arg and "a_" not in arg, f"floating-point atomic on {arg}"
 
 
-@pytest.mark.parametrize("model,float_t", [("cuba", "double"), ("cobahh", "double"), ("cuba_synw", "float")])
-def test_generated_kernels_compile_for_gfx950(model, float_t):
+@pytest.mark.parametrize("model,float_t,tile", [("cuba", "double", 256), ("cobahh", "double", 256),
+                                                ("cuba_synw", "float", 1024), ("cuba_synw", "double", 1024)])
+def test_generated_kernels_compile_for_gfx950(model, float_t, tile):
     m = Model.load(model)
     G = NeuronGroup(1000, m, float_t=float_t)
     sc = _scalars(G)
     fold = [(0, "gi", "+="), (1, "ge", "+=")]
     K = NeuronKernel("g", 1000, G.vars, float_t, G.su, G.th, G.rs, "fused" if G.rs else "none", sc, fold, 1024,
-                     "double", spikemon=(0, 1000), statemon=["v"], uses_refractory=G.uses_refractory)
+                     "double", spikemon=(0, 1000), statemon=["v"], uses_refractory=G.uses_refractory, tile=tile)
     _no_float_atomics(K.source)
     _compile(K.source)
     spec = m.pathways["exc"]
     syn = {"w": "float"} if model == "cuba_synw" else {}
+    n_tiles = -(-1000 // tile)
     gk = GatherKernel("p", 1, 50 if model == "cuba_synw" else 1, 1, spec.vector, syn,
-                      scalar_values(spec, {"dt": 1e-4}, float_t), "double", 1024, float_t, "ge")
+                      scalar_values(spec, {"dt": 1e-4}, float_t), "double", float_t, "ge", n_tiles, tile)
     _no_float_atomics(gk.source)
     _compile(gk.source)
+
+
+def test_gather_kernel_limits():
+    spec = Model.load("cuba").pathways["exc"]
+    sc = scalar_values(spec, {"dt": 1e-4}, "double")
+    with pytest.raises(NotImplementedError):
+        GatherKernel("p", 16, 1, 0, spec.vector, {}, sc, "double", "double", "ge", 4, 256)
+    with pytest.raises(NotImplementedError):
+        GatherKernel("p", 0, 5000, 0, spec.vector, {}, sc, "double", "double", "ge", 4, 256)

@@ -1,0 +1,51 @@
+"""Synchronous bursts (tests/bursts.py) through every path of the gather and the
+fold, against the sequential fp64 fold in the cython runtime's delivery order
+(itself checked against the oracle on the CPU).
+
+Launch shapes force multi-pass folds (LDS capacity 128 events), per-target
+global sorting (hub targets above half that capacity), more runs per step than
+threads per tile (the fold re-reads the runs), and tiles of 64 and 1024.
+"""
+import numpy as np
+import pytest
+
+from bursts import N_HUB, N_SRC, N_TGT, SPIKE_STEPS, STEPS, burst_case, expected_trace
+
+pytestmark = pytest.mark.gpu
+
+
+def device_trace(pre, post, w, d, **shape):
+    from brian2_amd import Network, NeuronGroup, SpikeGeneratorGroup, StateMonitor, Synapses
+    from test_gpu_kats import onpre_spec
+
+    idx = np.tile(np.arange(N_SRC), len(SPIKE_STEPS))
+    steps = np.repeat(np.array(SPIKE_STEPS), N_SRC)
+    src = SpikeGeneratorGroup(N_SRC, idx, timebins=steps)
+    tgt = NeuronGroup(N_TGT + N_HUB, variables={"v": "float64"})
+    S = Synapses(src, tgt, onpre_spec("w", ("w",)))
+    S.connect(pre, post)
+    S.set("w", w)
+    S.set_delay(d.astype(np.float64) * 1e-4)
+    mon = StateMonitor(tgt, ["v"], record=np.arange(N_TGT + N_HUB), when="end")
+    net = Network(src, tgt, S, mon, dt=1e-4, **shape)
+    net.run(STEPS)
+    return mon.values()[0]
+
+
+@pytest.fixture(scope="module")
+def case():
+    pre, post, w, d = burst_case()
+    return pre, post, w, d, expected_trace(pre, post, w, d)
+
+
+@pytest.mark.parametrize("shape", [
+    {},
+    {"tile": 64, "bucket_cap": 128, "gather_pairs": 32},
+    {"tile": 1024, "gather_pairs": 1024},
+    {"tile": 256, "bucket_cap": 512, "gather_pairs": 64},
+], ids=["default", "tile64-cap128-pairs32", "tile1024-pairs1024", "tile256-cap512-pairs64"])
+def test_burst_trace_bitexact(case, shape):
+    pre, post, w, d, exp = case
+    print(f"burst case, launch shape {shape}", flush=True)
+    got = device_trace(pre, post, w, d, **shape)
+    assert np.array_equal(got, exp)

@@ -34,12 +34,20 @@ def test_synw_device_csr_matches_oracle_generator():
         assert np.array_equal(dw, w32[sel][o])
 
 
-def test_synw_delays_bitexact_vs_brian():
+@pytest.mark.parametrize("shape", [
+    {},
+    {"tile": 64, "bucket_cap": 128, "gather_pairs": 32},
+    {"tile": 1024, "gather_pairs": 1024},
+], ids=["default", "tile64-cap128-pairs32", "tile1024-pairs1024"])
+def test_synw_delays_bitexact_vs_brian(shape):
+    """Launch shapes force multi-pass folds, per-target global sorting (in-degree
+    1000 above half of an LDS capacity of 128) and multi-window gather units."""
     from brian2_amd import synth
 
     z, meta = O.load_golden("cuba_delay")
+    print(f"cuba_delay golden, launch shape {shape}", flush=True)
     c = synth.cuba_synw_network(N=meta["N"], K=meta["K"], dmin=meta["dmin"], dmax=meta["dmax"], seed=meta["seed"],
-                                ginh=meta["ginh"], record=[0, 1, 2, 3])
+                                ginh=meta["ginh"], record=[0, 1, 2, 3], network_options=shape)
     c["net"].run(meta["n_steps"])
     assert np.array_equal(c["mon"].i, z["i"]) and np.array_equal(c["mon"].step, z["step"])
     final = np.stack([c["G"].get(n) for n in ("v", "ge", "gi")])

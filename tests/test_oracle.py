@@ -354,3 +354,28 @@ def _oracle_kat(case):
 def test_oracle_transmission_kats(name):
     case = kats.CASES[name]
     assert np.array_equal(_oracle_kat(case), kats.expected_matrix(case))
+
+
+def test_burst_fold_restatement_equals_oracle():
+    """The sequential fold of tests/bursts.py (used as the device expectation) is
+    the oracle's spike queue + passive target, value for value."""
+    import bursts as B
+
+    pre, post, w, d = B.burst_case()
+    exp = B.expected_trace(pre, post, w, d)
+    n_tgt = B.N_TGT + B.N_HUB
+    idx = np.tile(np.arange(B.N_SRC, dtype=np.int32), len(B.SPIKE_STEPS))
+    steps = np.repeat(np.array(B.SPIKE_STEPS, np.int32), B.N_SRC)
+    o = np.lexsort((idx, steps))
+    src = O.OracleGroup(O.HBR_SPIKEGEN, B.N_SRC, np.zeros((1, B.N_SRC)), [0.0], gen=(idx[o], steps[o]))
+    tgt = O.OracleGroup(O.HBR_PASSIVE, n_tgt, np.zeros((1, n_tgt)), [0.0])
+    path = O.OraclePathway(src, tgt, pre, post, 0, w=w.astype(np.float64), delay=d.astype(np.float64) * kats.DT)
+    _, _, rec = O.run([src, tgt], [path], kats.DT, B.STEPS, mon_group=None, rec_group=1,
+                      rec_idx=np.arange(n_tgt), rec_vars=[0])
+    start = rec[:, 0, :].T
+    got = np.concatenate([start[:, 1:], tgt.state[0][:, None]], axis=1)
+    assert np.array_equal(got, exp)
+    # a different order changes some value: sources reversed (every source spikes at the
+    # same steps), or the synapse order reversed (only multapses share source and delay)
+    assert not np.array_equal(exp, B.expected_trace(B.N_SRC - 1 - pre, post, w, d))
+    assert not np.array_equal(exp, B.expected_trace(pre[::-1], post[::-1], w[::-1], d[::-1]))
