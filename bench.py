@@ -3,13 +3,24 @@
 Workload (the 1M-neuron configuration): CUBA LIF, N = 1,000,000 (800k exc /
 200k inh), fixed in-degree 1000 drawn without replacement (1e9 synapses,
 int32 indices, fp32 weights), per-synapse delays U{1..50} steps, dt = 0.1 ms,
-fp64 state (bit-exact verification mode).  One "step" is one simulation time
-step of the whole network (state update, threshold, spike queue, propagation).
+fp64 state (bit-exact verification mode), 500 ms simulated.  One "step" is
+one simulation time step of the whole network (state update, threshold, spike
+queue, propagation).
 
-Prints one JSON line (metric, value, roofline, cpu_baseline, ...) on stdout;
-progress goes to stderr.
+Measurement: after W warm-up steps the network state is stored on the device,
+K steps are timed with hipGraph replay (the value), the state is restored and
+the same K steps are run again one launch at a time with HIP events around
+each launch on the plan's stream (per-kernel durations and the roofline; the
+simulation is deterministic, so both passes process identical events).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+Second leg: the batched sweep (1024 CUBA networks, N ~ U{500..8000}, swept
+excitatory weight): networks·simulated-seconds per second of the persistent
+batched kernel vs the plain per-network path on a sample of the same networks,
+with the sample's spikes checked identical.
+
+Prints one JSON line on stdout; progress goes to stderr.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-sweep] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -21,6 +32,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+DT = 1e-4
 
 
 def log(*a):
@@ -36,7 +48,7 @@ def cpu_baseline(n=20000, K=1000, warm=500, steps=5000):
     import oracle_py as O
 
     meta = dict(N=n, NE=n * 4 // 5, K=K, taum=0.02, taue=0.005, taui=0.01, El=-0.049, Vt=-0.05, Vr=-0.06,
-                refractory=0.005, dt=1e-4, dmin=1, dmax=50, seed=1234, we_base=1.62e-3 * 80.0 / K,
+                refractory=0.005, dt=DT, dmin=1, dmax=50, seed=1234, we_base=1.62e-3 * 80.0 / K,
                 wi_base=-9e-3 * 80.0 / K * 0.4)
     G, paths = O.build_cuba_delay(meta)
     for p in paths:
@@ -52,6 +64,66 @@ def cpu_baseline(n=20000, K=1000, warm=500, steps=5000):
                        f"{steps} steps after {warm} warm-up steps; {events} events in {wall:.1f} s")
 
 
+def kernel_bytes(kind, n, events, spikes, fbytes):
+    """Algorithmic bytes of one launch: neuron state read + write (v, ge, gi),
+    lastspike read, not_refractory read + write; per spike the ring entry, the
+    monitor record and the lastspike write; per event the post index, the fp32
+    weight and the delay code."""
+    if kind == "step":
+        return n * (3 * fbytes * 2 + 8 + 2) + spikes * (4 + 8 + 8)
+    return events * (4 + 4 + 1)
+
+
+def sweep_bench(n_nets=1024, steps=10000, sample=16, sample_steps=1000, float_t="double"):
+    """Batched sweep vs the plain per-network path (nets·sim-s/s)."""
+    import numpy as np
+    import torch
+
+    from brian2_amd import batched, synth
+
+    nets = synth.sweep_networks(n_nets=n_nets)
+    B = batched.BatchedCUBA(nets, float_t=float_t)
+    t0 = time.perf_counter()
+    B.build()
+    best, res = B.autotune(trial_steps=200)
+    log(f"sweep: {n_nets} networks ({sum(n.N for n in nets)} neurons, {B.n_syn} synapses) built and tuned in "
+        f"{time.perf_counter() - t0:.1f} s: block {best[0]}, {best[1]} steps per launch")
+    steps -= steps % B.steps_per_launch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.run(steps)
+    torch.cuda.synchronize()
+    wall_b = time.perf_counter() - t0
+    rate_b = n_nets * steps * DT / wall_b
+    log(f"sweep: batched {steps} steps in {wall_b:.2f} s -> {rate_b:.1f} nets*sim-s/s")
+    idx = np.unique(np.linspace(0, n_nets - 1, sample).astype(int))
+    wall_p, sim_p, same = 0.0, 0.0, True
+    for k in idx:
+        net = nets[k]
+        c = synth.cuba_network(N=net.N, NE=net.NE, seed=net.seed, we=net.namespace["we"], float_t=float_t)
+        c["net"].run(10)  # kernel compilation and graph capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        c["net"].run(sample_steps)
+        torch.cuda.synchronize()
+        wall_p += time.perf_counter() - t0
+        sim_p += sample_steps * DT
+        i, s = B.spikes(k)
+        sel = s < 10 + sample_steps
+        o = np.lexsort((i[sel], s[sel]))
+        same &= bool(np.array_equal(i[sel][o], c["mon"].i) and np.array_equal(s[sel][o], c["mon"].step))
+        del c
+    rate_p = sim_p / wall_p
+    log(f"sweep: plain path {len(idx)} networks x {sample_steps} steps in {wall_p:.2f} s -> {rate_p:.2f} "
+        f"nets*sim-s/s; spikes identical: {same}")
+    return {"metric": "nets*sim-s/s", "value": rate_b, "unit": "networks x simulated s / s",
+            "plain_value": rate_p, "vs_plain": rate_b / rate_p, "spikes_identical": same,
+            "dtype": "f64" if float_t == "double" else "f32", "n_nets": n_nets, "steps": steps,
+            "block": best[0], "steps_per_launch": best[1],
+            "plain_sample": f"{len(idx)} of the {n_nets} networks (evenly spaced), {sample_steps} steps each, "
+                            f"one Network per network, hipGraph replay of 100 steps"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,9 +132,10 @@ def main():
     ap.add_argument("--N", type=int, default=1_000_000)
     ap.add_argument("--K", type=int, default=1000)
     ap.add_argument("--graph-steps", type=int, default=100)
-    ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--float", dest="float_t", default="double", choices=["double", "float"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -75,54 +148,51 @@ def main():
 
     log(f"device: {torch.cuda.get_device_name(0)}")
     t0 = time.perf_counter()
-    c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=args.graph_steps,
-                                bucket_cap=None)
+    c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=args.graph_steps)
     torch.cuda.synchronize()
     log(f"built network: {c['n_syn']} synapses in {time.perf_counter() - t0:.1f} s")
-    net = c["net"]
+    net, mon = c["net"], c["mon"]
     t0 = time.perf_counter()
     net.run(args.warmup)
     torch.cuda.synchronize()
     log(f"warm-up {args.warmup} steps (includes kernel compilation, graph capture): {time.perf_counter() - t0:.2f} s")
+    net.store()
 
     ev0 = net.n_events
-    spk0 = c["mon"].mon_n.item()
+    spk0 = int(mon.mon_n.item())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     net.run(args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     events = net.n_events - ev0
-    spikes = c["mon"].mon_n.item() - spk0
+    spikes = int(mon.mon_n.item()) - spk0
     ms_per_step = wall / args.steps * 1e3
-    rate = spikes / args.N / (args.steps * 1e-4)
+    rate = spikes / args.N / (args.steps * DT)
     log(f"timed {args.steps} steps: {wall:.3f} s, {ms_per_step * 1e3:.1f} us/step, {events} events, "
         f"{spikes} spikes ({rate:.1f} Hz)")
 
-    # per-kernel durations (HIP events around each launch on the plan's stream)
+    # the same steps again, one launch at a time, HIP events around each launch
+    net.restore()
     ev_p0 = {s.name: s.n_events for s in net.paths}
-    spk_p0 = c["mon"].mon_n.item()
-    prof = net.profile(args.profile_steps)
+    prof = net.profile(args.steps)
+    net.run(0)
     ev_p = {s.name: s.n_events - ev_p0[s.name] for s in net.paths}
-    spk_p = c["mon"].mon_n.item() - spk_p0
-    P = args.profile_steps
+    same = sum(ev_p.values()) == events and int(mon.mon_n.item()) - spk0 == spikes
+    log(f"profiled pass: {sum(ev_p.values())} events, identical to the timed pass: {same}")
     fbytes = 8 if args.float_t == "double" else 4
-    b_neuron = 3 * fbytes * 2 + 8 + 2       # v, ge, gi read+write, lastspike read, not_refractory r/w
-    b_event = 4 + 4 + 1                     # post index, fp32 weight, delay code
     kernels = []
     for kind, name, ms in prof:
-        avg_ms = ms / P
-        if kind == "step":
-            alg = args.N * b_neuron + spk_p / P * 4
-        else:
-            alg = ev_p[name] / P * b_event
-        kernels.append(dict(kernel=f"{kind}_{name}", avg_us=avg_ms * 1e3, alg_bytes=alg,
-                            gbs=alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0))
+        avg_us = ms / args.steps * 1e3
+        alg = kernel_bytes(kind, args.N, ev_p.get(name, 0) / args.steps, spikes / args.steps, fbytes)
+        kernels.append(dict(kernel=f"{kind}_{name}", avg_us=avg_us, alg_bytes=alg,
+                            gbs=alg / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0))
     for k in kernels:
         log(f"  {k['kernel']}: {k['avg_us']:.1f} us/launch, {k['alg_bytes'] / 1e6:.1f} MB algorithmic, "
             f"{k['gbs']:.0f} GB/s")
     dom = max(kernels, key=lambda k: k["avg_us"])
-    step_bytes = args.N * b_neuron + events / args.steps * b_event + spikes / args.steps * 4
+    step_bytes = kernel_bytes("step", args.N, 0, spikes / args.steps, fbytes) + \
+        kernel_bytes("gather", 0, events / args.steps, 0, fbytes)
     step_frac = step_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
     log(f"whole step: {step_bytes / 1e6:.1f} MB algorithmic per step -> {step_frac * 100:.1f}% of HBM peak")
 
@@ -146,14 +216,22 @@ def main():
         "dtype": "f64" if args.float_t == "double" else "f32",
         "data": "synthetic",
         "config": {"workload": "CUBA LIF 1M neurons, fixed in-degree 1000 (1e9 synapses), delays 1-50 steps, "
-                               "fp32 weights, dt 0.1 ms (the 1M-neuron configuration)",
+                               "fp32 weights, dt 0.1 ms, 500 ms simulated (the 1M-neuron configuration)",
                    "N": args.N, "in_degree": args.K, "n_synapses": c["n_syn"], "sim_steps_timed": args.steps,
-                   "mean_rate_hz": rate, "parallelism": "single GPU"},
+                   "mean_rate_hz": rate, "spikes": spikes, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom["kernel"],
                      "avg_launch_us": dom["avg_us"], "step_frac": step_frac},
         "kernels": kernels,
+        "profiled_pass_identical": same,
     }
+    del c, net, mon
+    if not args.no_sweep:
+        import gc
+
+        gc.collect()
+        torch.cuda.empty_cache()
+        out["batched"] = sweep_bench(steps=args.sweep_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on a bounded sample) ...")
         out["cpu_baseline"] = cpu_baseline()

@@ -109,31 +109,41 @@ def _rename(lines, names):
     return [pat.sub(lambda m: names[m.group(1)], ln) for ln in lines]
 
 
-FOLD_EVENTS_RESIDENT = 4096  # fold: events of a tile kept in registers (spread over its threads)
+FOLD_EVENTS_PER_THREAD = 8    # fold: events a thread keeps in registers
 GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per window
+LDS_BYTES = 163840            # LDS of one CU
 
 
 def fold_capacity(T, val_t):
-    """Events of one tile sorted in LDS at a time, sized so that two workgroups of
-    T threads fit the 160 KiB of LDS of one CU."""
+    """Events of one tile sorted in LDS at a time.  A step kernel needs about 120
+    VGPRs, i.e. 16 waves per CU, so max(1, 1024 // T) workgroups of T threads
+    share the LDS of a CU; each gets an equal share."""
     vbytes = 8 if val_t in ("double", "long long") else 4
-    cap = (81920 - 16 * T - 1024) // (8 + vbytes)
-    return int(min(8192, cap - cap % 64))
+    per_wg = LDS_BYTES // max(1, 1024 // T)
+    xw = 2 * T * FOLD_EVENTS_PER_THREAD
+    cap = (per_wg - 16 * T - xw - 1024) // (8 + vbytes)
+    return int(max(128, min(16384, cap - cap % 64)))
 
 
 def _fold_source(T, cap, vt, flush):
     """Fold of the events staged by the previous step's gathers.
 
-    Phase A reads this tile's slice of every run (keys, values and positions kept
-    in registers when they fit) and counts events per target.  Phase B assigns the
-    targets to passes: all of them at once when the tile's events fit in LDS, else
-    by the position of their first event in windows of cap/2 (a pass then covers
-    at most cap events).  In each pass the events of its targets are scattered into
-    LDS, and the thread of each target sorts its run by key and applies it to the
+    Phase A reads this tile's slice of every run and counts events per target in
+    LDS.  Runs are taken T at a time (one row of the run table per thread); their
+    events are numbered in chunks of T*U, and each row writes its index over its
+    events of the chunk in LDS (so an event finds its run with one LDS read).
+    The first T*U events of the tile stay in registers (U per thread).  Phase B
+    computes each target's first event (block scan) and assigns the targets to
+    passes: all of them at once when the tile's events fit in LDS, else by the
+    position of their first event in windows of cap/2 (a pass then covers at most
+    cap events).  In each pass the events of its targets are scattered into LDS,
+    and the thread of each target sorts its run by key and applies it to the
     target's registers.  A target with more than cap/2 events in one step (an
-    in-degree above cap/2) is sorted through an index list in global memory."""
+    in-degree above cap/2) is sorted through an index list in global memory.
+    When a tile has more than T*U events, the passes read them again."""
     nw = T // 64
-    U = max(1, min(8, FOLD_EVENTS_RESIDENT // T))
+    U = FOLD_EVENTS_PER_THREAD
+    XW = T * U
     W = cap // 2
     k = [
         "    // ---- events delivered by propagate(step - 1): this tile's slice of every run ----",
@@ -141,6 +151,7 @@ def _fold_source(T, cap, vt, flush):
         "    (void)hb_par;",
         f"    __shared__ unsigned long long s_key[{cap}];",
         f"    __shared__ {vt} s_val[{cap}];",
+        f"    __shared__ unsigned short s_x[{XW}];",
         f"    __shared__ unsigned int s_lo[{T}];",
         f"    __shared__ int s_ro[{T}];",
         f"    __shared__ int s_beg[{T}];",
@@ -163,48 +174,64 @@ def _fold_source(T, cap, vt, flush):
         f"    unsigned long long hb_fk[{U}];",
         f"    {vt} hb_fv[{U}];",
         f"    unsigned int hb_fq[{U}];",
-        "    int hb_nres = -1;  // events held in registers (-1: re-read them from the runs)",
+        "    unsigned int hb_rlo, hb_rlen;",
+        "    int hb_rex, hb_btot;",
         "    int hb_tot = 0;",
-        # row batch loader (sets s_lo/s_ro, returns the batch's events and row count)
-        "#define HB_ROWS(r0, tot, nr) { \\",
-        "        const int r_ = (r0) + threadIdx.x; \\",
-        "        unsigned int lo_ = 0, len_ = 0; \\",
-        "        if (r_ < hb_R) { lo_ = p.run_tab[r_ * hb_rs + blockIdx.x]; "
-        "len_ = p.run_tab[r_ * hb_rs + blockIdx.x + 1] - lo_; } \\",
-        "        const int ex_ = hb_block_exscan((int)len_, s_sh, &(tot)); \\",
-        "        s_lo[threadIdx.x] = lo_; s_ro[threadIdx.x] = ex_; \\",
-        f"        nr = hb_R - (r0) < {T} ? hb_R - (r0) : {T}; \\",
+        # this thread's row of a batch of T rows; s_lo/s_ro of the batch
+        "#define HB_ROWS(r0) { \\",
+        "        const int r_ = (r0) + (int)threadIdx.x; \\",
+        "        hb_rlo = 0u; hb_rlen = 0u; \\",
+        "        if (r_ < hb_R) { hb_rlo = p.run_tab[r_ * hb_rs + blockIdx.x]; "
+        "hb_rlen = p.run_tab[r_ * hb_rs + blockIdx.x + 1] - hb_rlo; } \\",
+        "        hb_rex = hb_block_exscan((int)hb_rlen, s_sh, &hb_btot); \\",
+        "        s_lo[threadIdx.x] = hb_rlo; s_ro[threadIdx.x] = hb_rex; \\",
         "        __syncthreads(); }",
-        "#define HB_LOCATE(ev, nr, pos) { \\",
-        "        int a_ = 0, b_ = (nr); \\",
-        "        while (b_ - a_ > 1) { const int m_ = (a_ + b_) >> 1; if (s_ro[m_] <= (ev)) a_ = m_; else b_ = m_; } \\",
-        "        pos = s_lo[a_] + (unsigned int)((ev) - s_ro[a_]); }",
+        # s_x[k] = row (in the batch) of event c0 + k of the batch
+        "#define HB_EXPAND(c0) { \\",
+        "        const int a_ = hb_rex > (c0) ? hb_rex : (c0); \\",
+        "        const int e_ = hb_rex + (int)hb_rlen; \\",
+        f"        const int b_ = e_ < (c0) + {XW} ? e_ : (c0) + {XW}; \\",
+        "        for (int x_ = a_; x_ < b_; x_++) s_x[x_ - (c0)] = (unsigned short)threadIdx.x; \\",
+        "        __syncthreads(); }",
+        "#define HB_POS(c0, kk) (s_lo[s_x[kk]] + (unsigned int)((c0) + (kk) - s_ro[s_x[kk]]))",
         "    // phase A: events per target of this tile",
         f"    for (int r0 = 0; r0 < hb_R; r0 += {T}) {{",
-        "        int tot, nr;",
-        "        HB_ROWS(r0, tot, nr)",
-        f"        const bool res = r0 == 0 && hb_R <= {T} && tot <= {T * U};",
-        f"        for (int b = 0; b < tot; b += {T * U}) {{",
+        "        HB_ROWS(r0)",
+        f"        for (int c0 = 0; c0 < hb_btot; c0 += {XW}) {{",
+        f"            const int n = hb_btot - c0 < {XW} ? hb_btot - c0 : {XW};",
+        "            HB_EXPAND(c0)",
+        "            const int base = hb_tot + c0;",
+        "            // events held in registers (event g in slot g / T of thread g % T):",
         "            // positions first, then all loads in flight at once, then the counts",
         "            #pragma unroll",
         f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
-        f"                const int ev = b + hb_u * {T} + (int)threadIdx.x;",
-        "                const int evc = ev < tot ? ev : tot - 1;",
-        "                HB_LOCATE(evc, nr, hb_fq[hb_u])",
+        f"                const int kk = hb_u * {T} + (int)threadIdx.x - base;",
+        "                if (kk >= 0 && kk < n) hb_fq[hb_u] = HB_POS(c0, kk);",
         "            }",
         "            #pragma unroll",
         f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
-        "                hb_fk[hb_u] = p.stg_key[hb_fq[hb_u]];",
-        "                if (res) hb_fv[hb_u] = p.stg_val[hb_fq[hb_u]];",
+        f"                const int kk = hb_u * {T} + (int)threadIdx.x - base;",
+        "                const bool ok = kk >= 0 && kk < n;",
+        "                const unsigned int q = ok ? hb_fq[hb_u] : 0u;",
+        "                const unsigned long long k_ = p.stg_key[q];",
+        f"                const {vt} v_ = p.stg_val[q];",
+        "                if (ok) { hb_fk[hb_u] = k_; hb_fv[hb_u] = v_; }",
         "            }",
         "            #pragma unroll",
-        f"            for (int hb_u = 0; hb_u < {U}; hb_u++)",
-        f"                if (b + hb_u * {T} + (int)threadIdx.x < tot) atomicAdd(&s_fill[(int)(hb_fk[hb_u] >> {48})], 1);",
+        f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+        f"                const int kk = hb_u * {T} + (int)threadIdx.x - base;",
+        f"                if (kk >= 0 && kk < n) atomicAdd(&s_fill[(int)(hb_fk[hb_u] >> {KEY_TGT_SHIFT})], 1);",
+        "            }",
+        "            // events beyond the registers: counted here, read again by the passes",
+        f"            if (base + n > {T * U})",
+        f"                for (int kk = threadIdx.x; kk < n; kk += {T})",
+        f"                    if (base + kk >= {T * U}) atomicAdd(&s_fill[(int)(p.stg_key[HB_POS(c0, kk)] >> {KEY_TGT_SHIFT})], 1);",
+        "            __syncthreads();",
         "        }",
-        "        if (res) hb_nres = tot;",
-        "        hb_tot += tot;",
+        "        hb_tot += hb_btot;",
         "        __syncthreads();",
         "    }",
+        f"    const bool hb_res = hb_tot <= {T * U};",
         "    // phase B: first event of every target; passes",
         "    const int hb_c = s_fill[threadIdx.x];",
         "    int hb_tot2;",
@@ -221,7 +248,7 @@ def _fold_source(T, cap, vt, flush):
         "    if (hb_ovf && threadIdx.x == 0) atomicOr(p.status, 4);",
         "    unsigned int* hb_go = p.stg_ord + (size_t)blockIdx.x * (size_t)p.tile_cap;",
         "#define HB_SCATTER(kk, vv, pos) { \\",
-        f"        const int t_ = (int)((kk) >> {48}); \\",
+        f"        const int t_ = (int)((kk) >> {KEY_TGT_SHIFT}); \\",
         "        const int bt_ = s_beg[t_]; \\",
         "        if (bt_ >= 0) { \\",
         f"            if (hb_one || bt_ / {W} == hk_pass) {{ \\",
@@ -234,23 +261,27 @@ def _fold_source(T, cap, vt, flush):
 
 def _fold_passes(T, U, W, cap, vt, apply_lines, apply_global_lines):
     """Phase C of the fold (see _fold_source): passes over target ranges."""
+    XW = T * U
     k = [
         "    for (int hk_pass = 0; hk_pass < hb_np; hk_pass++) {",
         f"        const int hk_pb = hb_one ? 0 : hk_pass * {W};",
-        "        if (hb_nres >= 0) {",
+        "        if (hb_res) {",
         "            #pragma unroll",
         f"            for (int hb_u = 0; hb_u < {U}; hb_u++)",
-        f"                if (hb_u * {T} + (int)threadIdx.x < hb_nres) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], hb_fq[hb_u])",
+        f"                if (hb_u * {T} + (int)threadIdx.x < hb_tot) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], hb_fq[hb_u])",
         "        } else {",
         f"            for (int r0 = 0; r0 < hb_R; r0 += {T}) {{",
-        "                int tot, nr;",
-        "                HB_ROWS(r0, tot, nr)",
-        f"                for (int ev = threadIdx.x; ev < tot; ev += {T}) {{",
-        "                    unsigned int pos;",
-        "                    HB_LOCATE(ev, nr, pos)",
-        "                    const unsigned long long kk = p.stg_key[pos];",
-        f"                    const {vt} vv = p.stg_val[pos];",
-        "                    HB_SCATTER(kk, vv, pos)",
+        "                HB_ROWS(r0)",
+        f"                for (int c0 = 0; c0 < hb_btot; c0 += {XW}) {{",
+        f"                    const int n = hb_btot - c0 < {XW} ? hb_btot - c0 : {XW};",
+        "                    HB_EXPAND(c0)",
+        f"                    for (int kk = threadIdx.x; kk < n; kk += {T}) {{",
+        "                        const unsigned int pos = HB_POS(c0, kk);",
+        "                        const unsigned long long k_ = p.stg_key[pos];",
+        f"                        const {vt} v_ = p.stg_val[pos];",
+        "                        HB_SCATTER(k_, v_, pos)",
+        "                    }",
+        "                    __syncthreads();",
         "                }",
         "                __syncthreads();",
         "            }",
@@ -281,7 +312,8 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, apply_global_lines):
         "        }",
         "    }",
         "#undef HB_ROWS",
-        "#undef HB_LOCATE",
+        "#undef HB_EXPAND",
+        "#undef HB_POS",
         "#undef HB_SCATTER",
     ]
     return k
@@ -526,8 +558,8 @@ class GatherKernel:
             raise NotImplementedError(f"delay range of more than {MAX_SEG} steps")
         if n_tiles > 16384:
             raise NotImplementedError("more than 16384 target tiles")
-        if pairs & (pairs - 1) or not 32 <= pairs <= GATHER_BLOCK:
-            raise ValueError("pairs per unit must be a power of two in [32, GATHER_BLOCK]")
+        if not 1 <= pairs <= GATHER_BLOCK:
+            raise ValueError("pairs per unit must be in [1, GATHER_BLOCK]")
         self.name = name
         self.rank = rank
         self.n_seg = n_seg
@@ -602,6 +634,7 @@ class GatherKernel:
             f"    __shared__ int s_o[{PC}];",
             f"    __shared__ unsigned short s_sg[{PC}];",
             f"    __shared__ int s_hist[{NT}];",
+            f"    __shared__ unsigned short s_pid[{WN}];",
             f"    __shared__ int s_sh[{GT // 64 + 1}];",
             "    __shared__ unsigned long long s_base;",
             "    __shared__ int s_row;",
@@ -643,6 +676,14 @@ class GatherKernel:
             "        __syncthreads();",
             f"        for (int w0 = 0; w0 < E; w0 += {WN}) {{",
             f"            const int Ew = E - w0 < {WN} ? E - w0 : {WN};",
+            "            // each pair writes its index over its events of the window",
+            "            if ((int)threadIdx.x < nq) {",
+            "                const int a_ = ex > w0 ? ex : w0;",
+            "                const int e_ = ex + (int)len;",
+            f"                const int b_ = e_ < w0 + {WN} ? e_ : w0 + {WN};",
+            "                for (int x_ = a_; x_ < b_; x_++) s_pid[x_ - w0] = (unsigned short)threadIdx.x;",
+            "            }",
+            "            __syncthreads();",
             f"            int hb_gt[{U}];",
             f"            unsigned long long hb_gk[{U}];",
             f"            {VT} hb_gv[{U}];",
@@ -658,8 +699,7 @@ class GatherKernel:
             f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
             f"                const int ev = w0 + hb_u * {GT} + (int)threadIdx.x;",
             "                const int evc = ev < E ? ev : E - 1;",
-            "                int lo = 0, hi = nq;",
-            "                while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (s_o[m] <= evc) lo = m; else hi = m; }",
+            "                const int lo = s_pid[evc - w0];",
             "                hb_ge[hb_u] = s_a[lo] + (unsigned int)(evc - s_o[lo]);",
             "                hb_gs[hb_u] = s_sg[lo];",
             "                hb_gt[hb_u] = ev < E ? 0 : -1;",

@@ -578,9 +578,7 @@ class Network:
             pairs_max = nsrc * s.csr["n_seg"]
             # pairs per unit: about one window of events per unit on average
             avg = s.csr["n_syn"] / max(1, pairs_max)
-            pc = 32
-            while pc < GATHER_BLOCK and (2 * pc) * avg <= 0.8 * window:
-                pc *= 2
+            pc = int(min(GATHER_BLOCK, max(32, 0.85 * window / max(avg, 1e-9))))
             s.pairs = int(self.gather_pairs or pc)
             s.units_max = -(-pairs_max // s.pairs)
             rows += s.units_max + -(-int(s.csr["n_syn"]) // window)
@@ -808,6 +806,65 @@ class Network:
         buf = bytes(args)
         _lib.check(L.hb_plan_set_args(self._plan, index, buf, len(buf)), "hb_plan_set_args")
 
+    # -- store / restore (Network.store/restore, brian2/core/network.py:600-740) ------
+    def _mutable(self):
+        """Device tensors that a run changes, by (object, attribute)."""
+        out = [(self, "clock"), (self, "status")]
+        for g in self.groups:
+            out += [(g, "ring_spk"), (g, "ring_cnt")]
+            for a in ("spiked", "gen_state", "run_rows", "stg_top"):
+                if getattr(g, a, None) is not None:
+                    out.append((g, a))
+            m = getattr(g, "smon", None)
+            if m is not None:
+                out += [(m, "mon_n"), (m, "count_dev")]
+        for s in self.paths:
+            out.append((s, "events_dev"))
+            if getattr(s, "syn_count_dev", None) is not None:
+                out.append((s, "syn_count_dev"))
+        return out
+
+    def store(self, name="default"):
+        """Keeps a device copy of the network's state (neuron state, spike rings,
+        clock, counters, monitor positions) under `name`.  Between runs no event
+        is staged (every run ends by folding the last step's events), so the
+        staging area needs no copy."""
+        if not self._built:
+            self._build()
+        snap = {"t_step": self.t_step, "tensors": {}, "state": {}, "monitors": {}}
+        for o, a in self._mutable():
+            snap["tensors"][(id(o), a)] = getattr(o, a).clone()
+        for g in self.groups:
+            if isinstance(g, NeuronGroup):
+                snap["state"][id(g)] = {n: t.clone() for n, t in g._dev.items()}
+            m = getattr(g, "smon", None)
+            if m is not None:
+                snap["monitors"][id(m)] = (list(m._i), list(m._step), m._drained)
+            m = getattr(g, "stmon", None)
+            if m is not None:
+                snap["monitors"][id(m)] = list(m._rows)
+        if not hasattr(self, "_stored"):
+            self._stored = {}
+        self._stored[name] = snap
+
+    def restore(self, name="default"):
+        """Returns the network to the state kept by store(name)."""
+        snap = self._stored[name]
+        for o, a in self._mutable():
+            getattr(o, a).copy_(snap["tensors"][(id(o), a)])
+        for g in self.groups:
+            if isinstance(g, NeuronGroup):
+                for n, t in g._dev.items():
+                    t.copy_(snap["state"][id(g)][n])
+            m = getattr(g, "smon", None)
+            if m is not None:
+                i, s, d = snap["monitors"][id(m)]
+                m._i, m._step, m._drained = list(i), list(s), d
+            m = getattr(g, "stmon", None)
+            if m is not None:
+                m._rows = list(snap["monitors"][id(m)])
+        self.t_step = snap["t_step"]
+
     # -- run ------------------------------------------------------------------
     def run(self, n_steps=None, duration=None, graph_steps=None):
         """Advances the network by n_steps time steps (or duration seconds)."""
@@ -871,9 +928,18 @@ class Network:
         simulation).  Returns [(kind, name, total_ms)] per launch of the step."""
         if not self._built:
             self._build()
+        for g in self.groups:
+            m = getattr(g, "smon", None)
+            if m is not None:
+                self._ensure_monitor_room(g, m, int(n_steps))
+            if getattr(g, "stmon", None) is not None:
+                raise NotImplementedError("profile() of a network with a StateMonitor")
         ms = (ctypes.c_float * len(self._launches))()
         _lib.check(L.hb_plan_profile(self._plan, self.device.stream, int(n_steps), ms), "hb_plan_profile")
         self.t_step += int(n_steps)
+        st = int(self.status.cpu().item())
+        if st:
+            raise _lib.HipBrianError(self._status_message(st))
         out = []
         for obj in list(self.groups) + list(self.paths):
             kind = "gather" if isinstance(obj, Synapses) else "step"

@@ -72,3 +72,34 @@ def test_cuba_fp32_mode_error_reported():
     n32 = len(c["mon"].i)
     # fp32 mode is not bit-exact; the spike count stays close to the fp64 oracle's
     assert abs(n32 - n64) <= 0.05 * n64
+
+
+def test_store_restore_replays_bit_identically():
+    """Network.store/restore (brian2/core/network.py:600-740): a restored network
+    repeats the same spikes, state and event counts; profile() (one launch at a
+    time) gives the same results as graph replay."""
+    from brian2_amd import synth
+
+    z, meta = O.load_golden("cuba_delay")
+    c = synth.cuba_synw_network(N=meta["N"], K=meta["K"], seed=meta["seed"], ginh=meta["ginh"])
+    net, mon = c["net"], c["mon"]
+    net.run(300)
+    net.store()
+    n0 = len(mon.i)
+    ev0 = net.n_events
+    net.run(400)
+    a_i, a_s = mon.i[n0:], mon.step[n0:]
+    a_state = [c["G"].get(v) for v in ("v", "ge", "gi")]
+    a_ev = net.n_events - ev0
+    assert len(a_i) > 0
+    net.restore()
+    assert net.t_step == 300 and len(mon.i) == n0 and net.n_events == ev0
+    net.run(400)
+    assert np.array_equal(mon.i[n0:], a_i) and np.array_equal(mon.step[n0:], a_s)
+    assert all(np.array_equal(c["G"].get(v), x) for v, x in zip(("v", "ge", "gi"), a_state))
+    assert net.n_events - ev0 == a_ev
+    net.restore()
+    net.profile(400)
+    net.run(0)  # folds the events of the last profiled step
+    assert np.array_equal(mon.i[n0:], a_i) and np.array_equal(mon.step[n0:], a_s)
+    assert all(np.array_equal(c["G"].get(v), x) for v, x in zip(("v", "ge", "gi"), a_state))
