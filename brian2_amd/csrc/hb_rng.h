@@ -11,7 +11,7 @@
 #ifndef HB_RNG_H
 #define HB_RNG_H
 
-#ifdef __HIPCC__
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
 #define HB_RNG_FN static inline __host__ __device__
 #else
 #define HB_RNG_FN static inline

@@ -55,18 +55,23 @@ def uniform_int(h, n):
     return (((np.asarray(h, np.uint64) >> np.uint64(32)) * np.uint64(n)) >> np.uint64(32)).astype(np.int64)
 
 
-def polar_normal(seed, n):
+def polar_normal(seed, n, stream=S_NORMAL):
+    """Standard normals by Marsaglia's polar method: normal i uses the pairs
+    (seed, stream, i*64 + 2a), (..., i*64 + 2a + 1) for attempts a = 0, 1, ...
+    until 0 < s < 1 (vectorised over i; log is the shared hb_log)."""
     out = np.empty(n)
-    for i in range(n):
-        a = 0
-        while True:
-            u1 = 2.0 * float(uniform(seed, S_NORMAL, i * 64 + 2 * a)) - 1.0
-            u2 = 2.0 * float(uniform(seed, S_NORMAL, i * 64 + 2 * a + 1)) - 1.0
-            s = u1 * u1 + u2 * u2
-            a += 1
-            if 0.0 < s < 1.0:
-                break
-        out[i] = u1 * math.sqrt((-2.0 * L.hb_host_log(s)) / s)
+    pending = np.arange(n, dtype=np.uint64)
+    a = 0
+    while len(pending):
+        u1 = 2.0 * uniform(seed, stream, pending * np.uint64(64) + np.uint64(2 * a)) - 1.0
+        u2 = 2.0 * uniform(seed, stream, pending * np.uint64(64) + np.uint64(2 * a + 1)) - 1.0
+        s = u1 * u1 + u2 * u2
+        ok = (s > 0.0) & (s < 1.0)
+        so = s[ok]
+        lg = np.array([L.hb_host_log(float(x)) for x in so])
+        out[pending[ok].astype(np.int64)] = u1[ok] * np.sqrt((-2.0 * lg) / so)
+        pending = pending[~ok]
+        a += 1
     return out
 
 

@@ -58,6 +58,10 @@ double hbr_uniform(uint64_t seed, uint64_t stream, uint64_t ctr) {
     return (double)(hbr_rand64(seed, stream, ctr) >> 11) * 0x1p-53;
 }
 
+void hbr_uniform_fill(uint64_t seed, uint64_t stream, uint64_t ctr0, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; i++) out[i] = hbr_uniform(seed, stream, ctr0 + (uint64_t)i);
+}
+
 uint32_t hbr_uniform_int(uint64_t h, uint32_t n) {
     return (uint32_t)(((h >> 32) * (uint64_t)n) >> 32);
 }
@@ -358,6 +362,34 @@ static void cuba_update(const cuba_consts* c, hbr_group* g, double t, double dt)
     }
 }
 
+/* LIF with an exponential current, method='exact', no refractoriness
+ * (dv/dt = (I + bias - v)/tau_m, dI/dt = -I/tau_s, threshold v > theta, reset v = 0).
+ * params: tau_m tau_s theta bias.  State: v, I.  Statements: spec_lif.json. */
+typedef struct {
+    double lio1, lio2, lio3, lio4, theta;
+} lif_consts;
+
+static void lif_scalar(const double* P, double dt, lif_consts* c) {
+    double tau_m = P[0], tau_s = P[1], bias = P[3];
+    c->theta = P[2];
+    c->lio1 = hb_exp((-dt) / tau_s);
+    c->lio2 = bias - (bias * hb_exp((-dt) / tau_m));
+    c->lio3 = (((tau_s * ((-hb_exp(dt / tau_m)) + hb_exp(dt / tau_s))) * hb_exp((-dt) / tau_m)) * hb_exp((-dt) / tau_s)) /
+              (tau_m - tau_s);
+    c->lio4 = hb_exp((-dt) / tau_m);
+}
+
+static void lif_update(const lif_consts* c, hbr_group* g) {
+    int32_t N = g->N;
+    double *v = g->state, *I = g->state + N;
+    for (int32_t i = 0; i < N; i++) {
+        double _I = c->lio1 * I[i];
+        double _v = c->lio2 + ((c->lio3 * I[i]) + (c->lio4 * v[i]));
+        I[i] = _I;
+        v[i] = _v;
+    }
+}
+
 /* COBAHH (examples/COBAHH.py), method='exponential_euler'.
  * params: Cm gl El EK ENa g_na g_kd VT taue taui Ee Ei mV ms refractory
  * State: v, m, n, h, ge, gi.  Statements: spec_cobahh.json. */
@@ -450,6 +482,7 @@ static void hh_update(const hh_consts* c, hbr_group* g, double t, double dt) {
 /* ------------------------------------------------------------------------- */
 typedef struct {
     cuba_consts cuba;
+    lif_consts lif;
     hh_consts hh;
     int32_t* spikes;   /* _spikespace[0:n] */
     int32_t nspk;
@@ -464,6 +497,7 @@ int hbr_run(hbr_sim* S) {
         rt[gi].spikes = (int32_t*)malloc(sizeof(int32_t) * (size_t)(g->N > 0 ? g->N : 1));
         if (g->model == HBR_CUBA_EXACT) cuba_scalar(g->params, dt, &rt[gi].cuba);
         else if (g->model == HBR_COBAHH_EE) hh_scalar(g->params, dt, &rt[gi].hh);
+        else if (g->model == HBR_LIF_EXACT) lif_scalar(g->params, dt, &rt[gi].lif);
         else if (g->model != HBR_SPIKEGEN && g->model != HBR_PASSIVE) {
             free(rt);
             return fail("unknown model");
@@ -501,6 +535,7 @@ int hbr_run(hbr_sim* S) {
             hbr_group* g = &S->groups[gi];
             if (g->model == HBR_CUBA_EXACT) cuba_update(&rt[gi].cuba, g, t, dt);
             else if (g->model == HBR_COBAHH_EE) hh_update(&rt[gi].hh, g, t, dt);
+            else if (g->model == HBR_LIF_EXACT) lif_update(&rt[gi].lif, g);
         }
         /* 'thresholds': threshold.pyx:18-32 / spikegenerator.pyx:6-26 */
         for (int32_t gi = 0; gi < S->n_groups; gi++) {
@@ -518,14 +553,17 @@ int hbr_run(hbr_sim* S) {
                 g->gen_lastindex += R->nspk;
                 continue;
             }
-            double thr = g->model == HBR_CUBA_EXACT ? R->cuba.Vt : R->hh.thr;
+            double thr = g->model == HBR_CUBA_EXACT ? R->cuba.Vt : g->model == HBR_LIF_EXACT ? R->lif.theta : R->hh.thr;
             double* v = g->state;
             for (int32_t i = 0; i < g->N; i++) {
-                int cond = (v[i] > thr) && g->not_refractory[i];
+                /* without refractoriness the condition is v > thr alone (neurongroup.py:392-395) */
+                int cond = (v[i] > thr) && (!g->not_refractory || g->not_refractory[i]);
                 if (cond) {
                     R->spikes[R->nspk++] = i;
-                    g->not_refractory[i] = 0;
-                    g->lastspike[i] = t;
+                    if (g->not_refractory) {
+                        g->not_refractory[i] = 0;
+                        g->lastspike[i] = t;
+                    }
                 }
             }
         }
@@ -570,6 +608,9 @@ int hbr_run(hbr_sim* S) {
             if (g->model == HBR_CUBA_EXACT) {
                 double* v = g->state;
                 for (int32_t s = 0; s < rt[gi].nspk; s++) v[rt[gi].spikes[s]] = rt[gi].cuba.Vr;
+            } else if (g->model == HBR_LIF_EXACT) {
+                double* v = g->state;
+                for (int32_t s = 0; s < rt[gi].nspk; s++) v[rt[gi].spikes[s]] = 0.0;
             }
         }
     }

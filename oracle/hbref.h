@@ -18,6 +18,7 @@ extern "C" {
 /* ---- input generator (the inputs of the configurations) ------------------ */
 uint64_t hbr_rand64(uint64_t seed, uint64_t stream, uint64_t ctr);
 double hbr_uniform(uint64_t seed, uint64_t stream, uint64_t ctr);
+void hbr_uniform_fill(uint64_t seed, uint64_t stream, uint64_t ctr0, int64_t n, double* out);
 uint32_t hbr_uniform_int(uint64_t h, uint32_t n);
 int64_t hbr_bernoulli(uint64_t seed, uint64_t stream, int32_t i0, int32_t i1, int32_t n_post,
                       double p, int32_t* pre, int32_t* post, int64_t cap);

@@ -56,6 +56,7 @@ def lib():
         L.hbr_rand64.argtypes = [u64, u64, u64]
         L.hbr_uniform.restype = dbl
         L.hbr_uniform.argtypes = [u64, u64, u64]
+        L.hbr_uniform_fill.argtypes = [u64, u64, u64, i64, P(dbl)]
         L.hbr_uniform_int.restype = ctypes.c_uint32
         L.hbr_uniform_int.argtypes = [u64, ctypes.c_uint32]
         L.hbr_bernoulli.restype = i64
@@ -98,11 +99,13 @@ def ptr(a, t):
 # ---------------------------------------------------------------------------
 # generator helpers (the oracle's restatement of the input generator)
 S_CONN, S_V0, S_DELAY, S_W, S_INDEG, S_NORMAL = 1, 2, 3, 4, 5, 6
+S_RATE, S_POISSON, S_CONN_IN, S_CONN_REC, S_W_IN, S_W_REC, S_V0_LIF, S_TARGET = 9, 10, 11, 12, 13, 14, 15, 16
 
 
 def uniform(seed, stream, n, start=0):
-    L = lib()
-    return np.array([L.hbr_uniform(seed, stream, start + i) for i in range(n)])
+    out = np.empty(int(n))
+    lib().hbr_uniform_fill(seed, stream, start, int(n), ptr(out, dbl))
+    return out
 
 
 def bernoulli(seed, i0, i1, n_post, p, stream=S_CONN):
@@ -168,7 +171,7 @@ class Queue:
 
 # ---------------------------------------------------------------------------
 class OracleGroup:
-    def __init__(self, model, N, state, params, lastspike=None, not_refractory=None, gen=None):
+    def __init__(self, model, N, state, params, lastspike=None, not_refractory=None, gen=None, refractory=True):
         self.model = model
         self.N = N
         self.state = np.ascontiguousarray(state, np.float64)
@@ -176,6 +179,8 @@ class OracleGroup:
         self.lastspike = (np.full(N, -1e4) if lastspike is None else np.array(lastspike, np.float64))
         self.not_refractory = (np.ones(N, np.uint8) if not_refractory is None
                                else np.array(not_refractory, np.uint8))
+        if not refractory:  # no lastspike / not_refractory variables (refractory=False)
+            self.lastspike = self.not_refractory = None
         self.gen = gen  # (index, timebin) for spike generators
         self.gen_lastindex = 0
 
@@ -276,7 +281,7 @@ def build_cuba4000(meta):
     return G, [Ci, Ce]
 
 
-def polar_normal(seed, n):
+def polar_normal(seed, n, stream=S_NORMAL):
     L = lib()
     out = np.empty(n)
     import math
@@ -284,8 +289,8 @@ def polar_normal(seed, n):
     for i in range(n):
         a = 0
         while True:
-            u1 = 2.0 * L.hbr_uniform(seed, S_NORMAL, i * 64 + 2 * a) - 1.0
-            u2 = 2.0 * L.hbr_uniform(seed, S_NORMAL, i * 64 + 2 * a + 1) - 1.0
+            u1 = 2.0 * L.hbr_uniform(seed, stream, i * 64 + 2 * a) - 1.0
+            u2 = 2.0 * L.hbr_uniform(seed, stream, i * 64 + 2 * a + 1) - 1.0
             s = u1 * u1 + u2 * u2
             a += 1
             if 0.0 < s < 1.0:
@@ -337,6 +342,49 @@ def build_cuba_delay(meta):
     Ce = OraclePathway(G, G, pre[exc], post[exc], 1, w=w32[exc].astype(np.float64), delay=d[exc] * dt)
     Ci = OraclePathway(G, G, pre[~exc], post[~exc], 2, w=w32[~exc].astype(np.float64), delay=d[~exc] * dt)
     return G, [Ci, Ce]
+
+
+LIF = dict(N=1000, NE=800, n_in=200, tau_m=0.02, tau_s=0.005, theta=0.01, bias=0.0105, p=0.1,
+           rate_lo=5.0, rate_hi=50.0, w_scale=0.5e-3, n_steps=2000, n_readout=10)
+
+
+def lif_inputs(trials, seed=1234, c=LIF):
+    """The differentiable LIF configuration's inputs from the generator: input
+    rates r_i = 5 + 45 U Hz; Poisson trains (input i spikes at step k of trial b
+    iff U(S_POISSON, (b*n_in + i)*n_steps + k) < r_i dt); Bernoulli(0.1) input
+    and recurrent connectivity (CSR order: source, then target); weights
+    sigma*z (z standard normal, polar method; sigma = 0.5 mV/sqrt(expected
+    fan-in)) rounded to float32; v0 = theta * U per (trial, neuron); I0 = 0."""
+    dt = 1e-4
+    n_in, N, T = c["n_in"], c["N"], c["n_steps"]
+    rates = c["rate_lo"] + (c["rate_hi"] - c["rate_lo"]) * uniform(seed, S_RATE, n_in)
+    pre_in, post_in = bernoulli(seed, 0, n_in, N, c["p"], stream=S_CONN_IN)
+    pre_rec, post_rec = bernoulli(seed, 0, N, N, c["p"], stream=S_CONN_REC)
+    sigma = c["w_scale"] / np.sqrt(c["p"] * (n_in + N))
+    w_in = (sigma * polar_normal(seed, len(pre_in), S_W_IN)).astype(np.float32)
+    w_rec = (sigma * polar_normal(seed, len(pre_rec), S_W_REC)).astype(np.float32)
+    trains = []
+    for b in range(trials):
+        u = uniform(seed, S_POISSON, n_in * T, start=b * n_in * T).reshape(n_in, T)
+        ii, kk = np.nonzero(u < (rates * dt)[:, None])
+        o = np.lexsort((ii, kk))
+        trains.append((ii[o].astype(np.int32), kk[o].astype(np.int32)))
+    v0 = c["theta"] * uniform(seed, S_V0_LIF, trials * N).reshape(trials, N)
+    targets = 0.02 + 0.16 * uniform(seed, S_TARGET, c["n_readout"])
+    return dict(rates=rates, pre_in=pre_in, post_in=post_in, pre_rec=pre_rec, post_rec=post_rec, w_in=w_in,
+                w_rec=w_rec, trains=trains, v0=v0, targets=targets, sigma=sigma)
+
+
+def build_lif(inp, trial, c=LIF):
+    """One trial of the LIF network on the oracle; pathways in Brian's order
+    (syn_in_pre, then syn_rec_pre)."""
+    N = c["N"]
+    src = OracleGroup(HBR_SPIKEGEN, c["n_in"], np.zeros((1, c["n_in"])), [0.0], gen=inp["trains"][trial])
+    G = OracleGroup(HBR_LIF_EXACT, N, np.stack([inp["v0"][trial], np.zeros(N)]),
+                    [c["tau_m"], c["tau_s"], c["theta"], c["bias"]], refractory=False)
+    P_in = OraclePathway(src, G, inp["pre_in"], inp["post_in"], 1, w=inp["w_in"].astype(np.float64))
+    P_rec = OraclePathway(G, G, inp["pre_rec"], inp["post_rec"], 1, w=inp["w_rec"].astype(np.float64))
+    return src, G, [P_in, P_rec]
 
 
 def load_golden(name):

@@ -379,3 +379,20 @@ def test_burst_fold_restatement_equals_oracle():
     # same steps), or the synapse order reversed (only multapses share source and delay)
     assert not np.array_equal(exp, B.expected_trace(B.N_SRC - 1 - pre, post, w, d))
     assert not np.array_equal(exp, B.expected_trace(pre[::-1], post[::-1], w[::-1], d[::-1]))
+
+
+def test_oracle_lif_equals_brian_golden():
+    """The differentiable LIF configuration's forward pass (Poisson trains, two
+    current pathways, exact integration, no refractoriness): the oracle
+    reproduces Brian's cython run bit for bit (tests/golden/golden_lif.npz)."""
+    z, meta = O.load_golden("lif")
+    inp = O.lif_inputs(meta["trials"], meta["seed"])
+    for b in range(meta["trials"]):
+        src, G, paths = O.build_lif(inp, b)
+        si, ss, rec = O.run([src, G], paths, meta["dt"], meta["n_steps"], mon_group=1, rec_group=1,
+                            rec_idx=[0, 1, 2, 3], rec_vars=[0, 1])
+        sel = z["trial"] == b
+        assert len(si) == sel.sum() > 1000
+        assert np.array_equal(si, z["i"][sel]) and np.array_equal(ss, z["step"][sel])
+        assert np.array_equal(G.state, z["final"][b])
+        assert np.array_equal(np.transpose(rec, (1, 2, 0)), z["trace"][b])
