@@ -20,7 +20,10 @@ with the sample's spikes checked identical.
 
 Prints one JSON line on stdout; progress goes to stderr.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-sweep] [--no-cpu-baseline]
+Third leg: the differentiable LIF network (256 trials of 200 ms): forward and
+exact adjoint backward, trials x simulated seconds per second.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-sweep] [--no-adjoint] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -124,6 +127,34 @@ def sweep_bench(n_nets=1024, steps=10000, sample=16, sample_steps=1000, float_t=
                             f"one Network per network, hipGraph replay of 100 steps"}
 
 
+def adjoint_bench(trials=256):
+    """Differentiable LIF network: forward + exact adjoint backward over all trials."""
+    import torch
+
+    from brian2_amd.adjoint import LIFNetwork
+
+    t0 = time.perf_counter()
+    net = LIFNetwork(trials=trials)
+    loss, saved = net.forward(net.w_in0, net.w_rec0, net.c["tau_m"], net.c["tau_s"])
+    net.backward(saved)
+    torch.cuda.synchronize()
+    log(f"adjoint: {trials} trials built and warmed in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    loss, saved = net.forward(net.w_in0, net.w_rec0, net.c["tau_m"], net.c["tau_s"])
+    torch.cuda.synchronize()
+    t_f = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = net.backward(saved)
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    sim = trials * net.T * net.dt
+    log(f"adjoint: forward {t_f * 1e3:.1f} ms, backward {t_b * 1e3:.1f} ms for {trials} trials x {net.T} steps; "
+        f"loss {loss:.6g}, dL/dtau_m {g[2]:.6g}, dL/dtau_s {g[3]:.6g}")
+    return {"metric": "trials*sim-s/s (forward + adjoint backward)", "value": sim / (t_f + t_b),
+            "unit": "trials x simulated s / s", "trials": trials, "steps": net.T, "forward_ms": t_f * 1e3,
+            "backward_ms": t_b * 1e3, "ckpt_every": net.C, "loss": loss, "dtype": "f64 state, f32 weights"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +166,7 @@ def main():
     ap.add_argument("--float", dest="float_t", default="double", choices=["double", "float"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -232,6 +264,8 @@ def main():
         gc.collect()
         torch.cuda.empty_cache()
         out["batched"] = sweep_bench(steps=args.sweep_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on a bounded sample) ...")
         out["cpu_baseline"] = cpu_baseline()

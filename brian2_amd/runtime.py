@@ -2,9 +2,9 @@
 
 The objects mirror the Brian objects of the clock-driven step (NeuronGroup,
 Synapses with an on_pre pathway, SpikeGeneratorGroup, SpikeMonitor,
-StateMonitor, Network.run) and are driven by Brian's optimised statements:
-either captured specs (``brian2_amd/specs``) or, when brian2 is importable,
-the statements of the user's own objects (``brian2_amd.brian_device``).
+StateMonitor, Network.run) and are driven by Brian's optimised statements,
+in the JSON form that Brian's code generator receives them
+(``brian2_amd/specs``, captured from Brian by tests/golden/make_golden.py).
 
 State lives in device memory as torch tensors (one per variable, structure of
 arrays); kernels are generated per group/pathway, compiled with hiprtc and
