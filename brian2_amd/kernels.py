@@ -114,19 +114,20 @@ GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per w
 LDS_BYTES = 163840            # LDS of one CU
 
 
-def fold_capacity(T, val_t):
+def fold_capacity(T, val_t, npt=1):
     """Events of one tile sorted in LDS at a time.  A step kernel needs about 120
     VGPRs, i.e. 16 waves per CU, so max(1, 1024 // T) workgroups of T threads
     share the LDS of a CU; each gets an equal share."""
     vbytes = 8 if val_t in ("double", "long long") else 4
     per_wg = LDS_BYTES // max(1, 1024 // T)
     xw = 2 * T * FOLD_EVENTS_PER_THREAD
-    cap = (per_wg - 16 * T - xw - 1024) // (8 + vbytes)
+    cap = (per_wg - 8 * T * npt - 8 * T - xw - 1024) // (8 + vbytes)
     return int(max(128, min(16384, cap - cap % 64)))
 
 
-def _fold_source(T, cap, vt, flush):
-    """Fold of the events staged by the previous step's gathers.
+def _fold_source(T, cap, vt, flush, npt=1):
+    """Fold of the events staged by the previous step's gathers, for a tile of
+    T*npt targets (thread x owns targets q*T + x, q < npt).
 
     Phase A reads this tile's slice of every run and counts events per target in
     LDS.  Runs are taken T at a time (one row of the run table per thread); their
@@ -137,7 +138,7 @@ def _fold_source(T, cap, vt, flush):
     passes: all of them at once when the tile's events fit in LDS, else by the
     position of their first event in windows of cap/2 (a pass then covers at most
     cap events).  In each pass the events of its targets are scattered into LDS,
-    and the thread of each target sorts its run by key and applies it to the
+    and the owner of each target sorts its run by key and applies it to the
     target's registers.  A target with more than cap/2 events in one step (an
     in-degree above cap/2) is sorted through an index list in global memory.
     When a tile has more than T*U events, the passes read them again."""
@@ -145,6 +146,7 @@ def _fold_source(T, cap, vt, flush):
     U = FOLD_EVENTS_PER_THREAD
     XW = T * U
     W = cap // 2
+    TN = T * npt
     k = [
         "    // ---- events delivered by propagate(step - 1): this tile's slice of every run ----",
         "    const int hb_par = (int)(step & 1), hb_ppar = (int)((step - 1) & 1);",
@@ -154,8 +156,8 @@ def _fold_source(T, cap, vt, flush):
         f"    __shared__ unsigned short s_x[{XW}];",
         f"    __shared__ unsigned int s_lo[{T}];",
         f"    __shared__ int s_ro[{T}];",
-        f"    __shared__ int s_beg[{T}];",
-        f"    __shared__ int s_fill[{T}];",
+        f"    __shared__ int s_beg[{TN}];",
+        f"    __shared__ int s_fill[{TN}];",
         f"    __shared__ int s_sh[{nw + 1}];",
         "    __shared__ int s_R;",
         "    if (threadIdx.x == 0) {",
@@ -167,7 +169,7 @@ def _fold_source(T, cap, vt, flush):
               "        if (blockIdx.x == 0) { p.run_rows[hb_par] = 0; p.stg_top[hb_par] = 0ull; }"]
     k += [
         "    }",
-        "    s_fill[threadIdx.x] = 0;",
+        f"    for (int t_ = threadIdx.x; t_ < {TN}; t_ += {T}) s_fill[t_] = 0;",
         "    __syncthreads();",
         "    const int hb_R = s_R;",
         "    const size_t hb_rs = (size_t)p.n_tiles + 1;",
@@ -177,7 +179,6 @@ def _fold_source(T, cap, vt, flush):
         "    unsigned int hb_rlo, hb_rlen;",
         "    int hb_rex, hb_btot;",
         "    int hb_tot = 0;",
-        # this thread's row of a batch of T rows; s_lo/s_ro of the batch
         "#define HB_ROWS(r0) { \\",
         "        const int r_ = (r0) + (int)threadIdx.x; \\",
         "        hb_rlo = 0u; hb_rlen = 0u; \\",
@@ -186,7 +187,6 @@ def _fold_source(T, cap, vt, flush):
         "        hb_rex = hb_block_exscan((int)hb_rlen, s_sh, &hb_btot); \\",
         "        s_lo[threadIdx.x] = hb_rlo; s_ro[threadIdx.x] = hb_rex; \\",
         "        __syncthreads(); }",
-        # s_x[k] = row (in the batch) of event c0 + k of the batch
         "#define HB_EXPAND(c0) { \\",
         "        const int a_ = hb_rex > (c0) ? hb_rex : (c0); \\",
         "        const int e_ = hb_rex + (int)hb_rlen; \\",
@@ -232,17 +232,27 @@ def _fold_source(T, cap, vt, flush):
         "        __syncthreads();",
         "    }",
         f"    const bool hb_res = hb_tot <= {T * U};",
-        "    // phase B: first event of every target; passes",
-        "    const int hb_c = s_fill[threadIdx.x];",
+        "    // phase B: first event of every target (segments in (thread, q) order); passes",
+        f"    int hb_c[{npt}], hb_b[{npt}], hb_myp[{npt}];",
+        f"    bool hb_bigt[{npt}];",
+        "    int hb_ct = 0;",
+        "    #pragma unroll",
+        f"    for (int q = 0; q < {npt}; q++) {{ hb_c[q] = s_fill[q * {T} + threadIdx.x]; hb_b[q] = hb_ct; hb_ct += hb_c[q]; }}",
         "    int hb_tot2;",
-        "    const int hb_b = hb_block_exscan(hb_c, s_sh, &hb_tot2);",
+        "    const int hb_tb = hb_block_exscan(hb_ct, s_sh, &hb_tot2);",
         f"    const bool hb_one = hb_tot <= {cap};",
-        f"    const bool hb_bigt = !hb_one && hb_c > {W};",
         f"    const int hb_np = hb_one ? (hb_tot > 0 ? 1 : 0) : (hb_tot + {W - 1}) / {W};",
-        f"    const int hb_myp = hb_one ? 0 : hb_b / {W};",
-        "    s_beg[threadIdx.x] = hb_bigt ? ~hb_b : hb_b;",
-        "    s_fill[threadIdx.x] = 0;",
-        "    const bool hb_anybig = __syncthreads_or(hb_bigt) != 0;",
+        "    bool hb_anyb = false;",
+        "    #pragma unroll",
+        f"    for (int q = 0; q < {npt}; q++) {{",
+        "        hb_b[q] += hb_tb;",
+        f"        hb_bigt[q] = !hb_one && hb_c[q] > {W};",
+        f"        hb_myp[q] = hb_one ? 0 : hb_b[q] / {W};",
+        f"        s_beg[q * {T} + threadIdx.x] = hb_bigt[q] ? ~hb_b[q] : hb_b[q];",
+        f"        s_fill[q * {T} + threadIdx.x] = 0;",
+        "        hb_anyb = hb_anyb || hb_bigt[q];",
+        "    }",
+        "    const bool hb_anybig = __syncthreads_or(hb_anyb) != 0;",
         "    // bursts beyond the tile's index list (only when memory capped it) are reported, not folded",
         "    const bool hb_ovf = hb_anybig && (long long)hb_tot > p.tile_cap;",
         "    if (hb_ovf && threadIdx.x == 0) atomicOr(p.status, 4);",
@@ -259,8 +269,9 @@ def _fold_source(T, cap, vt, flush):
     return k, U, W
 
 
-def _fold_passes(T, U, W, cap, vt, apply_lines, apply_global_lines):
-    """Phase C of the fold (see _fold_source): passes over target ranges."""
+def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
+    """Phase C of the fold (see _fold_source): passes over target ranges.
+    `apply_lines` apply (hb_k, hb_v) to target q's registers."""
     XW = T * U
     k = [
         "    for (int hk_pass = 0; hk_pass < hb_np; hk_pass++) {",
@@ -287,28 +298,34 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, apply_global_lines):
         "            }",
         "        }",
         "        __syncthreads();",
-        "        if (!hb_bigt && hb_myp == hk_pass && hb_c > 0) {",
-        "            const int o = hb_b - hk_pb;",
-        "            hb_sort_kv(s_key + o, s_val + o, hb_c);",
-        "            for (int e = o; e < o + hb_c; e++) {",
-        "                const unsigned long long hb_k = s_key[e];",
-        f"                const {vt} hb_v = s_val[e];",
+        "        #pragma unroll",
+        f"        for (int q = 0; q < {npt}; q++) {{",
+        "            if (!hb_bigt[q] && hb_myp[q] == hk_pass && hb_c[q] > 0) {",
+        "                const int o = hb_b[q] - hk_pb;",
+        "                hb_sort_kv(s_key + o, s_val + o, hb_c[q]);",
+        "                for (int e = o; e < o + hb_c[q]; e++) {",
+        "                    const unsigned long long hb_k = s_key[e];",
+        f"                    const {vt} hb_v = s_val[e];",
     ]
-    k += ["                " + x for x in apply_lines]
+    k += ["                    " + x for x in apply_lines]
     k += [
+        "                }",
         "            }",
         "        }",
         "        __syncthreads();",
         "    }",
-        "    if (hb_bigt && !hb_ovf) {",
-        "        hb_sort_idx(hb_go + hb_b, p.stg_key, hb_c);",
-        "        for (int e = hb_b; e < hb_b + hb_c; e++) {",
-        "            const unsigned int r = hb_go[e];",
-        "            const unsigned long long hb_k = p.stg_key[r];",
-        f"            const {vt} hb_v = p.stg_val[r];",
+        "    #pragma unroll",
+        f"    for (int q = 0; q < {npt}; q++) {{",
+        "        if (hb_bigt[q] && !hb_ovf) {",
+        "            hb_sort_idx(hb_go + hb_b[q], p.stg_key, hb_c[q]);",
+        "            for (int e = hb_b[q]; e < hb_b[q] + hb_c[q]; e++) {",
+        "                const unsigned int r = hb_go[e];",
+        "                const unsigned long long hb_k = p.stg_key[r];",
+        f"                const {vt} hb_v = p.stg_val[r];",
     ]
-    k += ["            " + x for x in apply_global_lines]
+    k += ["                " + x for x in apply_lines]
     k += [
+        "            }",
         "        }",
         "    }",
         "#undef HB_ROWS",
@@ -326,7 +343,7 @@ class NeuronKernel:
     def __init__(self, name, N, arrays, float_t, stateupdate=None, threshold=None, reset=None,
                  reset_mode="none", scalars=None, fold=None, lds_cap=0, val_t="double",
                  spikemon=None, statemon=None, flush=False, uses_refractory=False, statemon_when="start",
-                 tile=256):
+                 tile=256, npt=1):
         self.name = name
         self.N = N
         self.arrays = arrays          # {var: ctype} of the group's state arrays
@@ -336,7 +353,7 @@ class NeuronKernel:
         self.scalars = scalars or {}  # {(tag, name): (value, ctype)}
         self.fold = fold or []        # [(rank, var, op)]
         self.val_t = val_t
-        self.cap = fold_capacity(tile, val_t) if not lds_cap else int(lds_cap)
+        self.cap = fold_capacity(tile, val_t, npt) if not lds_cap else int(lds_cap)
         if self.cap < 128 or self.cap % 2:
             raise ValueError("fold capacity must be an even number of at least 128 events")
         self.spikemon = spikemon      # (start, stop) or None
@@ -344,7 +361,10 @@ class NeuronKernel:
         self.statemon_when = statemon_when
         self.flush = flush
         self.uses_refractory = uses_refractory
-        self.T = tile
+        self.T = tile                 # threads per workgroup
+        self.npt = npt                # neurons per thread: a workgroup owns tile * npt neurons
+        if npt < 1 or tile * npt > 65536:
+            raise ValueError("a tile holds at most 65536 neurons")
         self._build()
 
     def _block(self, tag, spec, extra_types):
@@ -405,7 +425,8 @@ class NeuronKernel:
         self.source = self._source(ps)
 
     def _source(self, ps):
-        T = self.T
+        T, NPT = self.T, self.npt
+        TN = T * NPT
         nw = T // 64
         k = [source_header(T), ps.c_decl()]
         kname = f"{'flush' if self.flush else 'step'}_{self.name}"
@@ -415,28 +436,42 @@ class NeuronKernel:
         k.append("    const double t = (double)step * p.dt;")
         k.append("    const double dt = p.dt;")
         k.append("    const int N = p.N;")
-        k.append(f"    const int i = blockIdx.x * {T} + threadIdx.x;")
-        k.append("    const int _idx = i;")
-        k.append("    const bool hb_active = i < N;")
-        k.append("    bool hb_cond = false;")
-        k.append("    (void)t; (void)dt; (void)_idx;")
-        # state of this thread's neuron in registers (the fold updates them)
+        k.append(f"    const int hb_i0 = blockIdx.x * {TN} + threadIdx.x;  // neuron q*{T} + hb_i0 for q < {NPT}")
+        k.append("    (void)t; (void)dt;")
+        # state of this thread's neurons in registers (the fold updates them)
+        for v, ct in self.arrays.items():
+            k.append(f"    {'bool' if ct == 'bool' else ct} hb_a_{v}[{NPT}];")
+        k.append(f"    bool hb_condq[{NPT}];")
+        k.append("    #pragma unroll")
+        k.append(f"    for (int q = 0; q < {NPT}; q++) {{")
+        k.append(f"        const int i = hb_i0 + q * {T};")
+        k.append("        const bool act = i < N;")
         for v, ct in self.arrays.items():
             if ct == "bool":
-                k.append(f"    bool {v} = hb_active ? p.a_{v}[i] != 0 : false;")
+                k.append(f"        hb_a_{v}[q] = act ? p.a_{v}[i] != 0 : false;")
             else:
-                k.append(f"    {ct} {v} = hb_active ? p.a_{v}[i] : ({ct})0;")
+                k.append(f"        hb_a_{v}[q] = act ? p.a_{v}[i] : ({ct})0;")
+        k.append("        hb_condq[q] = false;")
+        k.append("    }")
         if self.fold:
-            fk, U, W = _fold_source(T, self.cap, self.val_t, self.flush)
+            fk, U, W = _fold_source(T, self.cap, self.val_t, self.flush, NPT)
             k += fk
             apply = ["switch ((int)((hb_k >> 44) & 15)) {"]
             for rank, var, op in self.fold:
-                apply.append(f"case {rank}: {var} {op} hb_v; break;")
+                apply.append(f"case {rank}: hb_a_{var}[q] {op} hb_v; break;")
             apply += ["default: break;", "}"]
-            k += _fold_passes(T, U, W, self.cap, self.val_t, apply, apply)
+            k += _fold_passes(T, U, W, self.cap, self.val_t, apply, NPT)
         # ---- per-neuron body ------------------------------------------------------
-        k.append("    if (hb_active) {")
+        k.append("    #pragma unroll")
+        k.append(f"    for (int q = 0; q < {NPT}; q++) {{")
+        k.append(f"        const int i = hb_i0 + q * {T};")
+        k.append("        const int _idx = i;")
+        k.append("        (void)_idx;")
+        k.append("        if (i < N) {")
         body = []
+        for v, ct in self.arrays.items():
+            body.append(f"{'bool' if ct == 'bool' else ct} {v} = hb_a_{v}[q];")
+        body.append("bool hb_cond = false;")
         if self.reset_mode == "deferred":
             body.append("if (p.spiked[i]) { // reset of the previous step (after its propagation)")
             body += ["    " + x for x in self._block("rs", self.rs, {})]
@@ -453,8 +488,8 @@ class NeuronKernel:
                 f"    const long long row = {row};",
                 "    if (lo < p.n_rec && p.rec_idx[lo] == i && row >= 0 && row < p.rec_rows) {",
             ]
-            for a, v in enumerate(self.statemon):
-                body.append(f"        p.rec_buf[((size_t)row * {nv} + {a}) * p.n_rec + lo] = (double){v};")
+            for a_, v in enumerate(self.statemon):
+                body.append(f"        p.rec_buf[((size_t)row * {nv} + {a_}) * p.n_rec + lo] = (double){v};")
             body += ["    }", "}"]
         if not self.flush:
             if self.su is not None:
@@ -480,58 +515,73 @@ class NeuronKernel:
             body.append("p.spiked[i] = 0;")
         for v, ct in self.arrays.items():
             body.append(f"p.a_{v}[i] = {v} ? 1 : 0;" if ct == "bool" else f"p.a_{v}[i] = {v};")
-        k += ["        " + x for x in body]
+        body.append("hb_condq[q] = hb_cond;")
+        k += ["            " + x for x in body]
+        k.append("        }")
         k.append("    }")
         if self.flush:
             k.append("}")
             return "\n".join(k) + "\n"
-        # ---- spikes of this step -> ring slot (ballot + workgroup prefix + one atomic)
+        # ---- spikes of this step -> ring slot (workgroup prefix + one atomic per workgroup)
         k += [
             "    {",
-            "        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;",
             "        const int slot = (int)(step % p.ring_slots);",
-            "        const unsigned long long bal = __ballot(hb_cond);",
-            "        const int rnk = __popcll(bal & ((1ull << lane) - 1ull));",
-            f"        __shared__ int s_w[{nw + 1}];",
+            f"        __shared__ int s_csh[{nw + 1}];",
             "        __shared__ int s_base;",
-            "        if (lane == 0) s_w[wid] = __popcll(bal);",
+            "        int hb_n = 0;",
+            "        #pragma unroll",
+            f"        for (int q = 0; q < {NPT}; q++) hb_n += hb_condq[q] ? 1 : 0;",
+            "        int hb_ntot;",
+            "        const int hb_off = hb_block_exscan(hb_n, s_csh, &hb_ntot);",
         ]
         if self.spikemon:
             k += [
-                "        const bool hb_m = hb_cond && i >= p.mon_start && i < p.mon_stop;",
-                "        const unsigned long long balm = __ballot(hb_m);",
-                "        const int rnkm = __popcll(balm & ((1ull << lane) - 1ull));",
-                f"        __shared__ int s_wm[{nw + 1}];",
+                f"        __shared__ int s_cshm[{nw + 1}];",
                 "        __shared__ unsigned long long s_mbase;",
-                "        if (lane == 0) s_wm[wid] = __popcll(balm);",
+                "        int hb_nm = 0;",
+                "        #pragma unroll",
+                f"        for (int q = 0; q < {NPT}; q++) {{",
+                f"            const int i = hb_i0 + q * {T};",
+                "            hb_nm += (hb_condq[q] && i >= p.mon_start && i < p.mon_stop) ? 1 : 0;",
+                "        }",
+                "        int hb_nmtot;",
+                "        const int hb_offm = hb_block_exscan(hb_nm, s_cshm, &hb_nmtot);",
             ]
         k += [
-            "        __syncthreads();",
             "        if (threadIdx.x == 0) {",
-            "            int acc = 0;",
-            f"            for (int w = 0; w < {nw}; w++) {{ const int c = s_w[w]; s_w[w] = acc; acc += c; }}",
-            "            s_base = acc ? atomicAdd(&p.ring_cnt[slot], acc) : 0;",
+            "            s_base = hb_ntot ? atomicAdd(&p.ring_cnt[slot], hb_ntot) : 0;",
         ]
         if self.spikemon:
             k += [
-                "            int accm = 0;",
-                f"            for (int w = 0; w < {nw}; w++) {{ const int c = s_wm[w]; s_wm[w] = accm; accm += c; }}",
-                "            s_mbase = accm ? atomicAdd(p.mon_n, (unsigned long long)accm) : 0ull;",
-                "            if (accm && s_mbase + (unsigned long long)accm > (unsigned long long)p.mon_cap) atomicOr(p.status, 2);",
+                "            s_mbase = hb_nmtot ? atomicAdd(p.mon_n, (unsigned long long)hb_nmtot) : 0ull;",
+                "            if (hb_nmtot && s_mbase + (unsigned long long)hb_nmtot > (unsigned long long)p.mon_cap) "
+                "atomicOr(p.status, 2);",
             ]
         k += [
             "        }",
             "        __syncthreads();",
-            "        if (hb_cond) p.ring_spk[(size_t)slot * p.ring_stride + s_base + s_w[wid] + rnk] = i;",
+            "        int hb_o = s_base + hb_off;",
+        ]
+        if self.spikemon:
+            k.append("        unsigned long long hb_om = s_mbase + (unsigned long long)hb_offm;")
+        k += [
+            "        #pragma unroll",
+            f"        for (int q = 0; q < {NPT}; q++) {{",
+            "            if (hb_condq[q]) {",
+            f"                const int i = hb_i0 + q * {T};",
+            "                p.ring_spk[(size_t)slot * p.ring_stride + hb_o] = i;",
+            "                hb_o++;",
         ]
         if self.spikemon:
             k += [
-                "        if (hb_m) {",
-                "            const unsigned long long q = s_mbase + (unsigned long long)(s_wm[wid] + rnkm);",
-                "            if (q < (unsigned long long)p.mon_cap) { p.mon_i[q] = i; p.mon_step[q] = (int)step; }",
-                "        }",
+                "                if (i >= p.mon_start && i < p.mon_stop) {",
+                "                    if (hb_om < (unsigned long long)p.mon_cap) { p.mon_i[hb_om] = i; p.mon_step[hb_om] = (int)step; }",
+                "                    hb_om++;",
+                "                }",
             ]
         k += [
+            "            }",
+            "        }",
             "        if (blockIdx.x == 0 && threadIdx.x == 0) p.ring_cnt[(int)((step + 1) % p.ring_slots)] = 0;",
             "    }",
             "}",

@@ -470,11 +470,12 @@ class Network:
     """Runs the step loop on the device (Network.run, brian2/core/network.py:1047-1292)."""
 
     def __init__(self, *objs, dt=1e-4, bucket_cap=None, graph_steps=100, bucket_mem_frac=0.3, tile=None,
-                 gather_pairs=None):
+                 gather_pairs=None, npt=None):
         """bucket_cap: events of a tile sorted in LDS at a time (default: what fits two
         workgroups per CU); bucket_mem_frac: share of free device memory the event
-        staging may use; tile: neurons per workgroup of the step kernels (default: 1024
-        for groups of at least 65536 neurons, else 256); gather_pairs: (delay, spike)
+        staging may use; tile: threads per workgroup of the step kernels (default:
+        1024 for groups of at least 65536 neurons, else 256); npt: neurons per thread
+        (default 1); gather_pairs: (delay, spike)
         pairs per gather unit (default: about one register window of events).  These
         shape the launches only: results do not depend on them."""
         self.objs = []
@@ -483,6 +484,7 @@ class Network:
         self.bucket_mem_frac = float(bucket_mem_frac)
         self.graph_steps = int(graph_steps)
         self.tile = tile
+        self.npt = npt
         self.gather_pairs = gather_pairs
         self.t_step = 0
         self._built = False
@@ -520,7 +522,10 @@ class Network:
             s._build_csr(self.dt)
         # per group: device state, spike ring
         for g in groups:
-            g.tile = int(self.tile or (1024 if g.N >= 65536 else 256))
+            # threads per workgroup, neurons per thread; a workgroup owns a tile of block * npt neurons
+            g.block = int(self.tile or (1024 if g.N >= 65536 else 256))
+            g.npt = int(self.npt or 1)
+            g.tile = g.block * g.npt
             dmax = max([s.csr["dmin"] + s.csr["n_seg"] - 1 for s in paths if s.source is g] + [0])
             g.ring_slots = dmax + 2
             g.ring_spk = zeros(g.ring_slots * g.N, "int")
@@ -569,7 +574,7 @@ class Network:
         T = g.tile
         g.n_tiles = (g.N + T - 1) // T
         vbytes = 8 if g.val_t in ("double", "long long") else 4
-        g.lds_cap = int(self.bucket_cap or fold_capacity(T, g.val_t))
+        g.lds_cap = int(self.bucket_cap or fold_capacity(g.block, g.val_t, g.npt))
         window = GATHER_BLOCK * GATHER_EVENTS_PER_THREAD
         total = sum(int(s.csr["n_syn"]) for s in incoming)
         rows = 0
@@ -671,7 +676,7 @@ class Network:
                          spikemon=g.smon.range if g.smon else None,
                          statemon=g.stmon.variables if g.stmon else None,
                          uses_refractory=g.uses_refractory,
-                         statemon_when=g.stmon.when if g.stmon else "start", tile=g.tile)
+                         statemon_when=g.stmon.when if g.stmon else "start", tile=g.block, npt=g.npt)
         g.kernel = K
         g.fn = D.function(K.source, K.kernel_name)
         g.struct_cls = K.struct.ctypes_struct()
@@ -689,13 +694,14 @@ class Network:
             m.rec_buf = None
         g.launch_index = len(self._launches)
         args = self._group_args(g)
-        self._add_launch(g.fn, (g.N + g.tile - 1) // g.tile, g.tile, args, g.struct_cls)
+        self._add_launch(g.fn, (g.N + g.tile - 1) // g.tile, g.block, args, g.struct_cls)
         end_mon = g.stmon is not None and g.stmon.when == "end"
         if g.incoming or mode == "deferred" or end_mon:
             F = NeuronKernel(g.name, g.N, g.vars, g.float_t, g.su, g.th, g.rs, mode, scalars, fold,
                              getattr(g, "lds_cap", 0), getattr(g, "val_t", "double"), flush=True,
                              uses_refractory=g.uses_refractory,
-                             statemon=g.stmon.variables if end_mon else None, statemon_when="end", tile=g.tile)
+                             statemon=g.stmon.variables if end_mon else None, statemon_when="end", tile=g.block,
+                             npt=g.npt)
             g.flush_fn = D.function(F.source, F.kernel_name)
             g.flush_struct_cls = F.struct.ctypes_struct()
         else:
@@ -907,7 +913,7 @@ class Network:
                 a.step_off = 0
                 buf = bytes(a)
                 grid = (g.N + g.tile - 1) // g.tile
-                _lib.check(L.hb_launch(g.flush_fn, grid, g.tile, 0, stream, buf, len(buf)), "flush")
+                _lib.check(L.hb_launch(g.flush_fn, grid, g.block, 0, stream, buf, len(buf)), "flush")
                 if getattr(g, "incoming", None):
                     g.run_rows.zero_()  # the flushed events are not folded again by the next run
         for g in self.groups:

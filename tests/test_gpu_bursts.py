@@ -43,7 +43,10 @@ def case():
     {"tile": 64, "bucket_cap": 128, "gather_pairs": 32},
     {"tile": 1024, "gather_pairs": 1024},
     {"tile": 256, "bucket_cap": 512, "gather_pairs": 64},
-], ids=["default", "tile64-cap128-pairs32", "tile1024-pairs1024", "tile256-cap512-pairs64"])
+    {"tile": 128, "npt": 4, "bucket_cap": 256, "gather_pairs": 48},
+    {"tile": 1024, "npt": 2},
+], ids=["default", "tile64-cap128-pairs32", "tile1024-pairs1024", "tile256-cap512-pairs64", "tile128-npt4-cap256",
+        "tile1024-npt2"])
 def test_burst_trace_bitexact(case, shape):
     pre, post, w, d, exp = case
     print(f"burst case, launch shape {shape}", flush=True)

@@ -38,7 +38,8 @@ def test_synw_device_csr_matches_oracle_generator():
     {},
     {"tile": 64, "bucket_cap": 128, "gather_pairs": 32},
     {"tile": 1024, "gather_pairs": 1024},
-], ids=["default", "tile64-cap128-pairs32", "tile1024-pairs1024"])
+    {"tile": 256, "npt": 3},
+], ids=["default", "tile64-cap128-pairs32", "tile1024-pairs1024", "tile256-npt3"])
 def test_synw_delays_bitexact_vs_brian(shape):
     """Launch shapes force multi-pass folds, per-target global sorting (in-degree
     1000 above half of an LDS capacity of 128) and multi-window gather units."""
