@@ -155,6 +155,27 @@ def adjoint_bench(trials=256):
             "backward_ms": t_b * 1e3, "ckpt_every": net.C, "loss": loss, "dtype": "f64 state, f32 weights"}
 
 
+def copy_calibration(nbytes=256 << 20, reps=3):
+    """A plain device-to-device copy of known size (hipMemcpyAsync), so that a
+    counter run (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) can scale the counters
+    of the simulation kernels by what they read for exactly nbytes."""
+    import ctypes
+
+    import torch
+
+    from brian2_amd import _lib
+
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for _ in range(reps):
+        _lib.check(_lib.lib.hb_memcpy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()), nbytes, 3,
+                                      stream), "copy")
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} device copies of {nbytes} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +188,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="start with device copies of known size (for counter runs)")
     ap.add_argument("--sweep-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -179,6 +202,8 @@ def main():
     from brian2_amd import synth
 
     log(f"device: {torch.cuda.get_device_name(0)}")
+    if args.copy_calibration:
+        copy_calibration()
     t0 = time.perf_counter()
     c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=args.graph_steps)
     torch.cuda.synchronize()

@@ -97,6 +97,11 @@ def linear_map(spec, consts, dt, float_t="double"):
     I' = beta I, with the coefficients and their derivatives with respect to
     tau_m and tau_s (autograd through the scalar block on the host)."""
     torch = _torch()
+    with torch.enable_grad():  # also when called from an autograd backward pass
+        return _linear_map(torch, spec, consts, dt)
+
+
+def _linear_map(torch, spec, consts, dt):
     import math
 
     tau = {n: torch.tensor(float(consts[n]), dtype=torch.float64, requires_grad=True) for n in ("tau_m", "tau_s")}
@@ -128,7 +133,7 @@ def linear_map(spec, consts, dt, float_t="double"):
     loc2.update(v=torch.tensor(1.0, dtype=torch.float64), I=torch.tensor(1.0, dtype=torch.float64))
     for st in spec.vector:
         loc2[st["var"]] = eval(st["expr"], {"__builtins__": {}}, {**fns, **loc2})
-    if not math.isclose(float(loc2["v"]), out["C"] + out["B"] + out["A"], rel_tol=1e-12, abs_tol=1e-15):
+    if not math.isclose(float(loc2["v"].detach()), out["C"] + out["B"] + out["A"], rel_tol=1e-12, abs_tol=1e-15):
         raise NotImplementedError("the neuron update is not linear in (v, I)")
     return out
 

@@ -231,7 +231,6 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "        hb_tot += hb_btot;",
         "        __syncthreads();",
         "    }",
-        f"    const bool hb_res = hb_tot <= {T * U};",
         "    // phase B: first event of every target (segments in (thread, q) order); passes",
         f"    int hb_c[{npt}], hb_b[{npt}], hb_myp[{npt}];",
         f"    bool hb_bigt[{npt}];",
@@ -253,6 +252,9 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "        hb_anyb = hb_anyb || hb_bigt[q];",
         "    }",
         "    const bool hb_anybig = __syncthreads_or(hb_anyb) != 0;",
+        "    // events held in registers are used as they are unless a target is sorted in global",
+        "    // memory (which needs their positions): then the passes read them again",
+        f"    const bool hb_res = hb_tot <= {T * U} && !hb_anybig;",
         "    // bursts beyond the tile's index list (only when memory capped it) are reported, not folded",
         "    const bool hb_ovf = hb_anybig && (long long)hb_tot > p.tile_cap;",
         "    if (hb_ovf && threadIdx.x == 0) atomicOr(p.status, 4);",
@@ -279,7 +281,7 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         "        if (hb_res) {",
         "            #pragma unroll",
         f"            for (int hb_u = 0; hb_u < {U}; hb_u++)",
-        f"                if (hb_u * {T} + (int)threadIdx.x < hb_tot) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], hb_fq[hb_u])",
+        f"                if (hb_u * {T} + (int)threadIdx.x < hb_tot) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], 0u)",
         "        } else {",
         f"            for (int r0 = 0; r0 < hb_R; r0 += {T}) {{",
         "                HB_ROWS(r0)",
