@@ -74,8 +74,10 @@ def test_adjoint_equals_autograd_of_the_tensor_reference():
 
 
 def test_adjoint_equals_central_finite_differences():
-    """fp64 central differences (relative step 1e-7 for the time constants,
-    1e-4 sigma for weights) at perturbations that leave every spike in place."""
+    """fp64 central differences at perturbations that leave every spike in place:
+    for each parameter the largest step of a short list (relative 1e-5 .. 1e-7 for
+    the time constants, 1e-2 .. 1e-4 sigma for weights) whose +h and -h runs
+    reproduce the unperturbed spike raster exactly."""
     net = _net(2, "double")
     w_in, w_rec = net.w_in0.clone(), net.w_rec0.clone()
     tm, ts = net.c["tau_m"], net.c["tau_s"]
@@ -89,26 +91,28 @@ def test_adjoint_equals_central_finite_differences():
         same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(sv["spikes"], ref))
         return loss, same
 
+    def fd(make, steps):
+        for h in steps:
+            (lp, sp), (lm, sm) = L(*make(h)), L(*make(-h))
+            if sp and sm:
+                return (lp - lm) / (2 * h), h
+        return None, None
+
     checks = []
-    h = 1e-7 * tm
-    (lp, sp), (lm, sm) = L(w_in, w_rec, tm + h, ts), L(w_in, w_rec, tm - h, ts)
-    checks.append(("tau_m", (lp - lm) / (2 * h), g_m, sp and sm))
-    h = 1e-7 * ts
-    (lp, sp), (lm, sm) = L(w_in, w_rec, tm, ts + h), L(w_in, w_rec, tm, ts - h)
-    checks.append(("tau_s", (lp - lm) / (2 * h), g_s, sp and sm))
-    hw = 1e-4 * net.inp["sigma"]
+    checks.append(("tau_m",) + fd(lambda h: (w_in, w_rec, tm + h, ts), [r * tm for r in (1e-5, 1e-6, 1e-7)]) + (g_m,))
+    checks.append(("tau_s",) + fd(lambda h: (w_in, w_rec, tm, ts + h), [r * ts for r in (1e-5, 1e-6, 1e-7)]) + (g_s,))
+    sig = net.inp["sigma"]
     for name, g, which in (("w_in", g_in, 0), ("w_rec", g_rec, 1)):
         for e in np.argsort(-np.abs(g))[:3]:
-            wp = [w_in.clone(), w_rec.clone()]
-            wm = [w_in.clone(), w_rec.clone()]
-            wp[which][int(e)] += hw
-            wm[which][int(e)] -= hw
-            (lp, sp), (lm, sm) = L(*wp, tm, ts), L(*wm, tm, ts)
-            checks.append((f"{name}[{e}]", (lp - lm) / (2 * hw), float(g[e]), sp and sm))
-    for name, fd, adj, same in checks:
-        print(f"{name}: finite difference {fd!r}, adjoint {adj!r}, spikes unchanged {same}", flush=True)
-        assert same, name
-        assert abs(fd - adj) <= 1e-6 * abs(adj), name
+            def make(h, e=int(e), which=which):
+                w = [w_in.clone(), w_rec.clone()]
+                w[which][e] += h
+                return w[0], w[1], tm, ts
+            checks.append((f"{name}[{e}]",) + fd(make, [r * sig for r in (1e-2, 1e-3, 1e-4)]) + (float(g[e]),))
+    for name, d, h, adj in checks:
+        print(f"{name}: finite difference {d!r} (step {h!r}), adjoint {adj!r}", flush=True)
+        assert d is not None, f"{name}: every step changed a spike"
+        assert abs(d - adj) <= 1e-6 * abs(adj), name
 
 
 def test_full_configuration_is_deterministic():
