@@ -110,6 +110,7 @@ def _rename(lines, names):
 
 
 FOLD_EVENTS_PER_THREAD = 8    # fold: events a thread keeps in registers
+RANK_SORT_MAX = 32            # fold: targets with at most this many events are ordered by rank counting
 GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per window
 LDS_BYTES = 163840            # LDS of one CU
 
@@ -122,7 +123,8 @@ def fold_capacity(T, val_t, npt=1):
     per_wg = LDS_BYTES // max(1, 1024 // T)
     xw = 2 * T * FOLD_EVENTS_PER_THREAD
     cap = (per_wg - 8 * T * npt - 8 * T - xw - 1024) // (8 + vbytes)
-    return int(max(128, min(16384, cap - cap % 64)))
+    # at most T*U: the expansion buffer holds a pass's order
+    return int(max(128, min(T * FOLD_EVENTS_PER_THREAD, cap - cap % 64)))
 
 
 def _fold_source(T, cap, vt, flush, npt=1):
@@ -300,17 +302,50 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         "            }",
         "        }",
         "        __syncthreads();",
+        "        if (hb_one) {",
+        "            // one pass: every event finds its rank among its target's events (keys are",
+        "            // distinct), one event per thread at a time; the order goes to s_x (free here)",
+        f"            for (int x = threadIdx.x; x < hb_tot; x += {T}) {{",
+        "                const unsigned long long kx = s_key[x];",
+        f"                const int t_ = (int)(kx >> {KEY_TGT_SHIFT});",
+        "                const int b_ = s_beg[t_], c_ = s_fill[t_];",
+        "                int r = 0;",
+        "                for (int j = 0; j < c_; j++) r += s_key[b_ + j] < kx ? 1 : 0;",
+        "                s_x[b_ + r] = (unsigned short)(x - b_);",
+        "            }",
+        "            __syncthreads();",
+        "        }",
         "        #pragma unroll",
         f"        for (int q = 0; q < {npt}; q++) {{",
         "            if (!hb_bigt[q] && hb_myp[q] == hk_pass && hb_c[q] > 0) {",
         "                const int o = hb_b[q] - hk_pb;",
-        "                hb_sort_kv(s_key + o, s_val + o, hb_c[q]);",
-        "                for (int e = o; e < o + hb_c[q]; e++) {",
-        "                    const unsigned long long hb_k = s_key[e];",
-        f"                    const {vt} hb_v = s_val[e];",
+        "                const int c = hb_c[q];",
+        f"                if (hb_one || c <= {RANK_SORT_MAX}) {{",
+        "                    if (!hb_one) { // several passes: the owner ranks its target's events",
+        "                        for (int a = 0; a < c; a++) {",
+        "                            const unsigned long long ka = s_key[o + a];",
+        "                            int r = 0;",
+        "                            for (int b2 = 0; b2 < c; b2++) r += s_key[o + b2] < ka ? 1 : 0;",
+        "                            s_x[o + r] = (unsigned short)a;",
+        "                        }",
+        "                    }",
+        "                    for (int r = 0; r < c; r++) {",
+        "                        const int a = o + (int)s_x[o + r];",
+        "                        const unsigned long long hb_k = s_key[a];",
+        f"                        const {vt} hb_v = s_val[a];",
     ]
-    k += ["                    " + x for x in apply_lines]
+    k += ["                        " + x for x in apply_lines]
     k += [
+        "                    }",
+        "                } else {",
+        "                    hb_sort_kv(s_key + o, s_val + o, c);",
+        "                    for (int e = o; e < o + c; e++) {",
+        "                        const unsigned long long hb_k = s_key[e];",
+        f"                        const {vt} hb_v = s_val[e];",
+    ]
+    k += ["                        " + x for x in apply_lines]
+    k += [
+        "                    }",
         "                }",
         "            }",
         "        }",
@@ -356,8 +391,8 @@ class NeuronKernel:
         self.fold = fold or []        # [(rank, var, op)]
         self.val_t = val_t
         self.cap = fold_capacity(tile, val_t, npt) if not lds_cap else int(lds_cap)
-        if self.cap < 128 or self.cap % 2:
-            raise ValueError("fold capacity must be an even number of at least 128 events")
+        if self.cap < 128 or self.cap % 2 or self.cap > tile * FOLD_EVENTS_PER_THREAD:
+            raise ValueError(f"fold capacity must be an even number of events in [128, {tile * FOLD_EVENTS_PER_THREAD}]")
         self.spikemon = spikemon      # (start, stop) or None
         self.statemon = statemon      # [vars] or None
         self.statemon_when = statemon_when
@@ -687,6 +722,13 @@ class GatherKernel:
             f"    __shared__ unsigned short s_sg[{PC}];",
             f"    __shared__ int s_hist[{NT}];",
             f"    __shared__ unsigned short s_pid[{WN}];",
+        ]
+        vb = 8 if VT in ("double", "long long") else 4
+        lds = WN * (8 + vb + 2) + NT * 4 + (NS + 1) * 4 + PC * 10 + 1024
+        self.staged = lds <= LDS_BYTES
+        if self.staged:
+            k += [f"    __shared__ unsigned long long s_ok[{WN}];", f"    __shared__ {VT} s_ov[{WN}];"]
+        k += [
             f"    __shared__ int s_sh[{GT // 64 + 1}];",
             "    __shared__ unsigned long long s_base;",
             "    __shared__ int s_row;",
@@ -808,21 +850,52 @@ class GatherKernel:
             "            }",
             f"            if (threadIdx.x == 0 && row < p.rows_cap) tab[{NT}] = ok ? (unsigned int)(base + (unsigned long long)Ew) : 0u;",
             "            __syncthreads();",
-            "            if (ok) {",
-            "                #pragma unroll",
-            f"                for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
-            "                    if (hb_gt[hb_u] >= 0) {",
-            "                        const unsigned long long dst = base + (unsigned long long)atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
-            "                        p.stg_key[dst] = hb_gk[hb_u];",
-            "                        p.stg_val[dst] = hb_gv[hb_u];",
         ]
-        if self.count_synapses:
-            k.append("                        p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")
+        if self.staged:
+            k += [
+                "            // the run sorted by tile in LDS, then written out contiguously (coalesced)",
+                "            if (ok) {",
+                "                #pragma unroll",
+                f"                for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+                "                    if (hb_gt[hb_u] >= 0) {",
+                "                        const int q = atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
+                "                        s_ok[q] = hb_gk[hb_u];",
+                "                        s_ov[q] = hb_gv[hb_u];",
+            ]
+            if self.count_synapses:
+                k.append("                        p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")
+            k += [
+                "                    }",
+                "                }",
+                "            }",
+                "            __syncthreads();",
+                "            if (ok) {",
+                f"                for (int x = threadIdx.x; x < Ew; x += {GT}) {{",
+                "                    p.stg_key[base + (unsigned long long)x] = s_ok[x];",
+                "                    p.stg_val[base + (unsigned long long)x] = s_ov[x];",
+                "                }",
+                "            }",
+                "            __syncthreads();",
+            ]
+        else:
+            k += [
+                "            if (ok) {",
+                "                #pragma unroll",
+                f"                for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+                "                    if (hb_gt[hb_u] >= 0) {",
+                "                        const unsigned long long dst = base + (unsigned long long)atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
+                "                        p.stg_key[dst] = hb_gk[hb_u];",
+                "                        p.stg_val[dst] = hb_gv[hb_u];",
+            ]
+            if self.count_synapses:
+                k.append("                        p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")
+            k += [
+                "                    }",
+                "                }",
+                "            }",
+                "            __syncthreads();",
+            ]
         k += [
-            "                    }",
-            "                }",
-            "            }",
-            "            __syncthreads();",
             f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
             "            __syncthreads();",
             "        }",

@@ -20,6 +20,7 @@ import numpy as np
 from . import _lib
 from .codegen import ctype_of
 from .kernels import (
+    FOLD_EVENTS_PER_THREAD,
     GATHER_BLOCK,
     GATHER_EVENTS_PER_THREAD,
     CodeSpec,
@@ -574,7 +575,8 @@ class Network:
         T = g.tile
         g.n_tiles = (g.N + T - 1) // T
         vbytes = 8 if g.val_t in ("double", "long long") else 4
-        g.lds_cap = int(self.bucket_cap or fold_capacity(g.block, g.val_t, g.npt))
+        g.lds_cap = int(min(self.bucket_cap, g.block * FOLD_EVENTS_PER_THREAD) if self.bucket_cap
+                        else fold_capacity(g.block, g.val_t, g.npt))
         window = GATHER_BLOCK * GATHER_EVENTS_PER_THREAD
         total = sum(int(s.csr["n_syn"]) for s in incoming)
         rows = 0
