@@ -335,7 +335,9 @@ extern "C" int hb_plan_profile(hb_plan* p, void* stream, int64_t n_steps, float*
     hipStream_t s = c->stream;
     const size_t nl = p->launches.size();
     std::vector<hipEvent_t> ev(nl + 1);
-    for (auto& e : ev) HB_HIP(hipEventCreate(&e));
+    // timing-only events: no system-scope fence (its cache writeback would be timed as part of
+    // the kernel before it)
+    for (auto& e : ev) HB_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     for (size_t k = 0; k < nl; k++) ms_per_launch[k] = 0.0f;
     HB_HIP(hipEventRecord(c->ev_in, (hipStream_t)stream));
     HB_HIP(hipStreamWaitEvent(s, c->ev_in, 0));
