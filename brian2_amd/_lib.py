@@ -53,6 +53,7 @@ _SIGS = {
     "hb_plan_create": ([_vp, _vp, _P(_vp)], _i),
     "hb_plan_add": ([_vp, _vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _vp, _sz, ctypes.c_long], _i),
     "hb_plan_set_args": ([_vp, _i, _vp, _sz], _i),
+    "hb_plan_set_lane": ([_vp, _i, _i], _i),
     "hb_plan_run": ([_vp, _vp, _i64, _i], _i),
     "hb_plan_destroy": ([_vp], _i),
     "hb_plan_profile": ([_vp, _vp, _i64, _P(ctypes.c_float)], _i),

@@ -111,6 +111,7 @@ def _rename(lines, names):
 
 FOLD_EVENTS_PER_THREAD = 8    # fold: events a thread keeps in registers
 RANK_SORT_MAX = 32            # fold: targets with at most this many events are ordered by rank counting
+XCD_TILES = True              # step kernels: consecutive tiles on one XCD
 GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per window
 LDS_BYTES = 163840            # LDS of one CU
 
@@ -122,7 +123,7 @@ def fold_capacity(T, val_t, npt=1):
     vbytes = 8 if val_t in ("double", "long long") else 4
     per_wg = LDS_BYTES // max(1, 1024 // T)
     xw = 2 * T * FOLD_EVENTS_PER_THREAD
-    cap = (per_wg - 8 * T * npt - 8 * T - xw - 1024) // (8 + vbytes)
+    cap = (per_wg - 8 * T * npt - 16 * T - xw - 1024) // (8 + vbytes)
     # at most T*U: the expansion buffer holds a pass's order
     return int(max(128, min(T * FOLD_EVENTS_PER_THREAD, cap - cap % 64)))
 
@@ -151,53 +152,65 @@ def _fold_source(T, cap, vt, flush, npt=1):
     TN = T * npt
     k = [
         "    // ---- events delivered by propagate(step - 1): this tile's slice of every run ----",
-        "    const int hb_par = (int)(step & 1), hb_ppar = (int)((step - 1) & 1);",
-        "    (void)hb_par;",
+        "    // staging buffers rotate over three steps: step s's gathers write buffer s % 3 (possibly",
+        "    // while this kernel runs), this kernel reads buffer (s - 1) % 3 and clears (s + 1) % 3",
+        "    const int hb_nxt = (int)((step + 1) % 3), hb_prv = (int)((step + 2) % 3);",
+        "    (void)hb_nxt;",
+        "    const unsigned long long* hb_sk = p.stg_key + (size_t)hb_prv * p.stg_stride;",
+        f"    const {vt}* hb_sv = p.stg_val + (size_t)hb_prv * p.stg_stride;",
+        "    const unsigned int* hb_tab = p.run_tab + (size_t)hb_prv * p.tab_stride;",
         f"    __shared__ unsigned long long s_key[{cap}];",
         f"    __shared__ {vt} s_val[{cap}];",
         f"    __shared__ unsigned short s_x[{XW}];",
-        f"    __shared__ unsigned int s_lo[{T}];",
-        f"    __shared__ int s_ro[{T}];",
+        f"    __shared__ unsigned int s_lo[{2 * T}];",
+        f"    __shared__ int s_ro[{2 * T}];",
         f"    __shared__ int s_beg[{TN}];",
         f"    __shared__ int s_fill[{TN}];",
         f"    __shared__ int s_sh[{nw + 1}];",
         "    __shared__ int s_R;",
+        "    const size_t hb_rs = (size_t)p.n_tiles + 1;",
         "    if (threadIdx.x == 0) {",
-        "        const int r = p.run_rows[hb_ppar];",
+        "        const int r = p.run_rows[hb_prv];",
         "        s_R = r < p.rows_cap ? r : p.rows_cap;",
     ]
     if not flush:
         k += ["        // the counters of this step's gathers (read by the next step's fold)",
-              "        if (blockIdx.x == 0) { p.run_rows[hb_par] = 0; p.stg_top[hb_par] = 0ull; }"]
+              "        if (blockIdx.x == 0) { p.run_rows[hb_nxt] = 0; p.stg_top[hb_nxt] = 0ull; }"]
     k += [
         "    }",
         f"    for (int t_ = threadIdx.x; t_ < {TN}; t_ += {T}) s_fill[t_] = 0;",
         "    __syncthreads();",
         "    const int hb_R = s_R;",
-        "    const size_t hb_rs = (size_t)p.n_tiles + 1;",
         f"    unsigned long long hb_fk[{U}];",
         f"    {vt} hb_fv[{U}];",
         f"    unsigned int hb_fq[{U}];",
-        "    unsigned int hb_rlo, hb_rlen;",
+        "    unsigned int hb_rlo0, hb_rlen0, hb_rlo1, hb_rlen1;",
         "    int hb_rex, hb_btot;",
         "    int hb_tot = 0;",
+        # rows r0 + x and r0 + T + x of thread x (the second only where it exists);
+        # s_lo/s_ro by row in the batch
         "#define HB_ROWS(r0) { \\",
-        "        const int r_ = (r0) + (int)threadIdx.x; \\",
-        "        hb_rlo = 0u; hb_rlen = 0u; \\",
-        "        if (r_ < hb_R) { hb_rlo = p.run_tab[r_ * hb_rs + blockIdx.x]; "
-        "hb_rlen = p.run_tab[r_ * hb_rs + blockIdx.x + 1] - hb_rlo; } \\",
-        "        hb_rex = hb_block_exscan((int)hb_rlen, s_sh, &hb_btot); \\",
-        "        s_lo[threadIdx.x] = hb_rlo; s_ro[threadIdx.x] = hb_rex; \\",
+        "        const int ra_ = (r0) + (int)threadIdx.x, rb_ = ra_ + " + str(T) + "; \\",
+        "        hb_rlo0 = hb_rlo1 = 0u; hb_rlen0 = hb_rlen1 = 0u; \\",
+        "        if (ra_ < hb_R) { hb_rlo0 = hb_tab[ra_ * hb_rs + hb_tile]; "
+        "hb_rlen0 = hb_tab[ra_ * hb_rs + hb_tile + 1] - hb_rlo0; } \\",
+        "        if (rb_ < hb_R) { hb_rlo1 = hb_tab[rb_ * hb_rs + hb_tile]; "
+        "hb_rlen1 = hb_tab[rb_ * hb_rs + hb_tile + 1] - hb_rlo1; } \\",
+        "        hb_rex = hb_block_exscan((int)(hb_rlen0 + hb_rlen1), s_sh, &hb_btot); \\",
+        "        s_lo[threadIdx.x] = hb_rlo0; s_ro[threadIdx.x] = hb_rex; \\",
+        "        s_lo[threadIdx.x + " + str(T) + "] = hb_rlo1; s_ro[threadIdx.x + " + str(T) + "] = hb_rex + (int)hb_rlen0; \\",
         "        __syncthreads(); }",
+        # s_x[k] = row (in the batch) of event c0 + k of the batch
         "#define HB_EXPAND(c0) { \\",
         "        const int a_ = hb_rex > (c0) ? hb_rex : (c0); \\",
-        "        const int e_ = hb_rex + (int)hb_rlen; \\",
+        "        const int e_ = hb_rex + (int)(hb_rlen0 + hb_rlen1); \\",
         f"        const int b_ = e_ < (c0) + {XW} ? e_ : (c0) + {XW}; \\",
-        "        for (int x_ = a_; x_ < b_; x_++) s_x[x_ - (c0)] = (unsigned short)threadIdx.x; \\",
+        "        const int m_ = hb_rex + (int)hb_rlen0; \\",
+        "        for (int x_ = a_; x_ < b_; x_++) s_x[x_ - (c0)] = (unsigned short)(threadIdx.x + (x_ >= m_ ? " + str(T) + " : 0)); \\",
         "        __syncthreads(); }",
         "#define HB_POS(c0, kk) (s_lo[s_x[kk]] + (unsigned int)((c0) + (kk) - s_ro[s_x[kk]]))",
         "    // phase A: events per target of this tile",
-        f"    for (int r0 = 0; r0 < hb_R; r0 += {T}) {{",
+        f"    for (int r0 = 0; r0 < hb_R; r0 += {2 * T}) {{",
         "        HB_ROWS(r0)",
         f"        for (int c0 = 0; c0 < hb_btot; c0 += {XW}) {{",
         f"            const int n = hb_btot - c0 < {XW} ? hb_btot - c0 : {XW};",
@@ -215,8 +228,8 @@ def _fold_source(T, cap, vt, flush, npt=1):
         f"                const int kk = hb_u * {T} + (int)threadIdx.x - base;",
         "                const bool ok = kk >= 0 && kk < n;",
         "                const unsigned int q = ok ? hb_fq[hb_u] : 0u;",
-        "                const unsigned long long k_ = p.stg_key[q];",
-        f"                const {vt} v_ = p.stg_val[q];",
+        "                const unsigned long long k_ = hb_sk[q];",
+        f"                const {vt} v_ = hb_sv[q];",
         "                if (ok) { hb_fk[hb_u] = k_; hb_fv[hb_u] = v_; }",
         "            }",
         "            #pragma unroll",
@@ -227,7 +240,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "            // events beyond the registers: counted here, read again by the passes",
         f"            if (base + n > {T * U})",
         f"                for (int kk = threadIdx.x; kk < n; kk += {T})",
-        f"                    if (base + kk >= {T * U}) atomicAdd(&s_fill[(int)(p.stg_key[HB_POS(c0, kk)] >> {KEY_TGT_SHIFT})], 1);",
+        f"                    if (base + kk >= {T * U}) atomicAdd(&s_fill[(int)(hb_sk[HB_POS(c0, kk)] >> {KEY_TGT_SHIFT})], 1);",
         "            __syncthreads();",
         "        }",
         "        hb_tot += hb_btot;",
@@ -260,7 +273,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "    // bursts beyond the tile's index list (only when memory capped it) are reported, not folded",
         "    const bool hb_ovf = hb_anybig && (long long)hb_tot > p.tile_cap;",
         "    if (hb_ovf && threadIdx.x == 0) atomicOr(p.status, 4);",
-        "    unsigned int* hb_go = p.stg_ord + (size_t)blockIdx.x * (size_t)p.tile_cap;",
+        "    unsigned int* hb_go = p.stg_ord + (size_t)hb_tile * (size_t)p.tile_cap;",
         "#define HB_SCATTER(kk, vv, pos) { \\",
         f"        const int t_ = (int)((kk) >> {KEY_TGT_SHIFT}); \\",
         "        const int bt_ = s_beg[t_]; \\",
@@ -274,93 +287,115 @@ def _fold_source(T, cap, vt, flush, npt=1):
 
 
 def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
-    """Phase C of the fold (see _fold_source): passes over target ranges.
-    `apply_lines` apply (hb_k, hb_v) to target q's registers."""
+    """Phase C of the fold (see _fold_source).  `apply_lines` apply (hb_k, hb_v) to
+    target q's registers.
+
+    One pass (the tile's events fit in LDS): events are scattered by target and
+    each event finds its rank among its target's (event-parallel rank counting).
+    Several passes: per pass, the owners order their targets by rank counting
+    (or heapsort above RANK_SORT_MAX events).  Targets sorted in global memory
+    come last."""
     XW = T * U
-    k = [
-        "    for (int hk_pass = 0; hk_pass < hb_np; hk_pass++) {",
-        f"        const int hk_pb = hb_one ? 0 : hk_pass * {W};",
-        "        if (hb_res) {",
-        "            #pragma unroll",
-        f"            for (int hb_u = 0; hb_u < {U}; hb_u++)",
-        f"                if (hb_u * {T} + (int)threadIdx.x < hb_tot) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], 0u)",
-        "        } else {",
-        f"            for (int r0 = 0; r0 < hb_R; r0 += {T}) {{",
-        "                HB_ROWS(r0)",
-        f"                for (int c0 = 0; c0 < hb_btot; c0 += {XW}) {{",
-        f"                    const int n = hb_btot - c0 < {XW} ? hb_btot - c0 : {XW};",
-        "                    HB_EXPAND(c0)",
-        f"                    for (int kk = threadIdx.x; kk < n; kk += {T}) {{",
-        "                        const unsigned int pos = HB_POS(c0, kk);",
-        "                        const unsigned long long k_ = p.stg_key[pos];",
-        f"                        const {vt} v_ = p.stg_val[pos];",
-        "                        HB_SCATTER(k_, v_, pos)",
-        "                    }",
-        "                    __syncthreads();",
-        "                }",
-        "                __syncthreads();",
-        "            }",
+
+    def scatter(ind):
+        return [ind + x for x in [
+            "if (hb_res) {",
+            "    #pragma unroll",
+            f"    for (int hb_u = 0; hb_u < {U}; hb_u++)",
+            f"        if (hb_u * {T} + (int)threadIdx.x < hb_tot) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], 0u)",
+            "} else {",
+            f"    for (int r0 = 0; r0 < hb_R; r0 += {2 * T}) {{",
+            "        HB_ROWS(r0)",
+            f"        for (int c0 = 0; c0 < hb_btot; c0 += {XW}) {{",
+            f"            const int n = hb_btot - c0 < {XW} ? hb_btot - c0 : {XW};",
+            "            HB_EXPAND(c0)",
+            f"            for (int kk = threadIdx.x; kk < n; kk += {T}) {{",
+            "                const unsigned int pos = HB_POS(c0, kk);",
+            "                const unsigned long long k_ = hb_sk[pos];",
+            f"                const {vt} v_ = hb_sv[pos];",
+            "                HB_SCATTER(k_, v_, pos)",
+            "            }",
+            "            __syncthreads();",
+            "        }",
+            "        __syncthreads();",
+            "    }",
+            "}",
+            "__syncthreads();",
+        ]]
+
+    def apply(ind, k_expr, v_expr):
+        return ([ind + "{", f"{ind}    const unsigned long long hb_k = {k_expr};", f"{ind}    const {vt} hb_v = {v_expr};"]
+                + [ind + "    " + x for x in apply_lines] + [ind + "}"])
+
+    k = ["    if (hb_one) {", "        const int hk_pass = 0, hk_pb = 0;", "        (void)hk_pass; (void)hk_pb;"]
+    k += scatter("        ")
+    k += [
+        "        // each event finds its rank among its target's (keys are distinct), one event per",
+        "        // thread at a time; the order goes to s_x",
+        f"        for (int x = threadIdx.x; x < hb_tot; x += {T}) {{",
+        "            const unsigned long long kx = s_key[x];",
+        f"            const int t_ = (int)(kx >> {KEY_TGT_SHIFT});",
+        "            const int b_ = s_beg[t_], c_ = s_fill[t_];",
+        "            int r = 0;",
+        "            for (int j = 0; j < c_; j++) r += s_key[b_ + j] < kx ? 1 : 0;",
+        "            s_x[b_ + r] = (unsigned short)(x - b_);",
         "        }",
         "        __syncthreads();",
-        "        if (hb_one) {",
-        "            // one pass: every event finds its rank among its target's events (keys are",
-        "            // distinct), one event per thread at a time; the order goes to s_x (free here)",
-        f"            for (int x = threadIdx.x; x < hb_tot; x += {T}) {{",
-        "                const unsigned long long kx = s_key[x];",
-        f"                const int t_ = (int)(kx >> {KEY_TGT_SHIFT});",
-        "                const int b_ = s_beg[t_], c_ = s_fill[t_];",
-        "                int r = 0;",
-        "                for (int j = 0; j < c_; j++) r += s_key[b_ + j] < kx ? 1 : 0;",
-        "                s_x[b_ + r] = (unsigned short)(x - b_);",
-        "            }",
-        "            __syncthreads();",
-        "        }",
         "        #pragma unroll",
         f"        for (int q = 0; q < {npt}; q++) {{",
-        "            if (!hb_bigt[q] && hb_myp[q] == hk_pass && hb_c[q] > 0) {",
-        "                const int o = hb_b[q] - hk_pb;",
-        "                const int c = hb_c[q];",
-        f"                if (hb_one || c <= {RANK_SORT_MAX}) {{",
-        "                    if (!hb_one) { // several passes: the owner ranks its target's events",
+        "            const int o = hb_b[q], c = hb_c[q];",
+        "            for (int r = 0; r < c; r++) {",
+        "                const int a = o + (int)s_x[o + r];",
+    ]
+    k += apply("                ", "s_key[a]", "s_val[a]")
+    k += [
+        "            }",
+        "        }",
+        "    } else {",
+        "        for (int hk_pass = 0; hk_pass < hb_np; hk_pass++) {",
+        f"            const int hk_pb = hk_pass * {W};",
+    ]
+    k += scatter("            ")
+    k += [
+        "            #pragma unroll",
+        f"            for (int q = 0; q < {npt}; q++) {{",
+        "                if (!hb_bigt[q] && hb_myp[q] == hk_pass && hb_c[q] > 0) {",
+        "                    const int o = hb_b[q] - hk_pb;",
+        "                    const int c = hb_c[q];",
+        f"                    if (c <= {RANK_SORT_MAX}) {{",
+        "                        // the owner ranks its target's events (order to s_x, free here)",
         "                        for (int a = 0; a < c; a++) {",
         "                            const unsigned long long ka = s_key[o + a];",
         "                            int r = 0;",
         "                            for (int b2 = 0; b2 < c; b2++) r += s_key[o + b2] < ka ? 1 : 0;",
         "                            s_x[o + r] = (unsigned short)a;",
         "                        }",
-        "                    }",
-        "                    for (int r = 0; r < c; r++) {",
-        "                        const int a = o + (int)s_x[o + r];",
-        "                        const unsigned long long hb_k = s_key[a];",
-        f"                        const {vt} hb_v = s_val[a];",
+        "                        for (int r = 0; r < c; r++) {",
+        "                            const int a = o + (int)s_x[o + r];",
     ]
-    k += ["                        " + x for x in apply_lines]
+    k += apply("                            ", "s_key[a]", "s_val[a]")
     k += [
-        "                    }",
-        "                } else {",
-        "                    hb_sort_kv(s_key + o, s_val + o, c);",
-        "                    for (int e = o; e < o + c; e++) {",
-        "                        const unsigned long long hb_k = s_key[e];",
-        f"                        const {vt} hb_v = s_val[e];",
+        "                        }",
+        "                    } else {",
+        "                        hb_sort_kv(s_key + o, s_val + o, c);",
+        "                        for (int e = o; e < o + c; e++)",
     ]
-    k += ["                        " + x for x in apply_lines]
+    k += apply("                            ", "s_key[e]", "s_val[e]")
     k += [
         "                    }",
         "                }",
         "            }",
+        "            __syncthreads();",
         "        }",
-        "        __syncthreads();",
         "    }",
         "    #pragma unroll",
         f"    for (int q = 0; q < {npt}; q++) {{",
         "        if (hb_bigt[q] && !hb_ovf) {",
-        "            hb_sort_idx(hb_go + hb_b[q], p.stg_key, hb_c[q]);",
+        "            hb_sort_idx(hb_go + hb_b[q], hb_sk, hb_c[q]);",
         "            for (int e = hb_b[q]; e < hb_b[q] + hb_c[q]; e++) {",
         "                const unsigned int r = hb_go[e];",
-        "                const unsigned long long hb_k = p.stg_key[r];",
-        f"                const {vt} hb_v = p.stg_val[r];",
     ]
-    k += ["                " + x for x in apply_lines]
+    k += apply("                ", "hb_sk[r]", "hb_sv[r]")
     k += [
         "            }",
         "        }",
@@ -440,6 +475,8 @@ class NeuronKernel:
             ps.val("rows_cap", "int")
             ps.val("n_tiles", "int")
             ps.val("tile_cap", "long long")
+            ps.val("stg_stride", "unsigned long long")
+            ps.val("tab_stride", "unsigned long long")
         if self.reset_mode == "deferred":
             ps.ptr("spiked", "unsigned char")
         ps.ptr("status", "int")
@@ -473,7 +510,21 @@ class NeuronKernel:
         k.append("    const double t = (double)step * p.dt;")
         k.append("    const double dt = p.dt;")
         k.append("    const int N = p.N;")
-        k.append(f"    const int hb_i0 = blockIdx.x * {TN} + threadIdx.x;  // neuron q*{T} + hb_i0 for q < {NPT}")
+        if XCD_TILES:
+            k += [
+                "    // tiles of consecutive neurons go to one XCD (workgroups are dealt to the 8 XCDs in turn),",
+                "    // so the run-table lines and staging slices neighbouring tiles share stay in one L2",
+                "    int hb_tile;",
+                "    {",
+                "        const int G_ = gridDim.x, q_ = G_ / 8, r_ = G_ % 8, x_ = blockIdx.x % 8, k_ = blockIdx.x / 8;",
+                "        hb_tile = x_ < r_ ? x_ * (q_ + 1) + k_ : r_ * (q_ + 1) + (x_ - r_) * q_ + k_;",
+                "    }",
+            ]
+        else:
+            k.append("    const int hb_tile = blockIdx.x;")
+        k += [
+            f"    const int hb_i0 = hb_tile * {TN} + threadIdx.x;  // neuron q*{T} + hb_i0 for q < {NPT}",
+        ]
         k.append("    (void)t; (void)dt;")
         # state of this thread's neurons in registers (the fold updates them)
         for v, ct in self.arrays.items():
@@ -687,6 +738,8 @@ class GatherKernel:
         ps.val("stg_cap", "unsigned long long")
         ps.ptr("run_tab", "unsigned int")
         ps.ptr("run_rows", "int")
+        ps.val("stg_stride", "unsigned long long")
+        ps.val("tab_stride", "unsigned long long")
         ps.ptr("status", "int")
         ps.ptr("events", "unsigned long long")  # per-workgroup delivered-event totals
         if self.count_synapses:
@@ -715,7 +768,9 @@ class GatherKernel:
         k.append(f'extern "C" __global__ void __launch_bounds__({GT}) {self.kernel_name}(const {ps.name} p) {{')
         k += [
             "    const long long step = p.clock[0] + p.step_off;",
-            "    const int par = (int)(step & 1);",
+            "    const int par = (int)(step % 3);  // staging buffer of this step",
+            "    unsigned long long* hb_sk = p.stg_key + (size_t)par * p.stg_stride;",
+            f"    {VT}* hb_sv = p.stg_val + (size_t)par * p.stg_stride;",
             f"    __shared__ int s_cum[{NS + 1}];",
             f"    __shared__ unsigned int s_a[{PC}];",
             f"    __shared__ int s_o[{PC}];",
@@ -834,7 +889,7 @@ class GatherKernel:
             "            const int row = s_row;",
             "            const bool ok = base + (unsigned long long)Ew <= p.stg_cap && row < p.rows_cap;",
             "            if (threadIdx.x == 0 && !ok) atomicOr(p.status, 1);",
-            f"            unsigned int* tab = p.run_tab + (size_t)row * {NT + 1};",
+            f"            unsigned int* tab = p.run_tab + (size_t)par * p.tab_stride + (size_t)row * {NT + 1};",
             "            // run-table row: absolute start of every tile's events (all empty if they do not fit)",
             "            int carry = 0;",
             f"            for (int t0 = 0; t0 < {NT}; t0 += {GT}) {{",
@@ -871,8 +926,8 @@ class GatherKernel:
                 "            __syncthreads();",
                 "            if (ok) {",
                 f"                for (int x = threadIdx.x; x < Ew; x += {GT}) {{",
-                "                    p.stg_key[base + (unsigned long long)x] = s_ok[x];",
-                "                    p.stg_val[base + (unsigned long long)x] = s_ov[x];",
+                "                    hb_sk[base + (unsigned long long)x] = s_ok[x];",
+                "                    hb_sv[base + (unsigned long long)x] = s_ov[x];",
                 "                }",
                 "            }",
                 "            __syncthreads();",
@@ -884,8 +939,8 @@ class GatherKernel:
                 f"                for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
                 "                    if (hb_gt[hb_u] >= 0) {",
                 "                        const unsigned long long dst = base + (unsigned long long)atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
-                "                        p.stg_key[dst] = hb_gk[hb_u];",
-                "                        p.stg_val[dst] = hb_gv[hb_u];",
+                "                        hb_sk[dst] = hb_gk[hb_u];",
+                "                        hb_sv[dst] = hb_gv[hb_u];",
             ]
             if self.count_synapses:
                 k.append("                        p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")

@@ -470,13 +470,14 @@ class StateMonitor:
 class Network:
     """Runs the step loop on the device (Network.run, brian2/core/network.py:1047-1292)."""
 
-    def __init__(self, *objs, dt=1e-4, bucket_cap=None, graph_steps=100, bucket_mem_frac=0.3, tile=None,
-                 gather_pairs=None, npt=None):
+    def __init__(self, *objs, dt=1e-4, bucket_cap=None, graph_steps=100, bucket_mem_frac=0.4, tile=None,
+                 gather_pairs=None, npt=None, overlap=True):
         """bucket_cap: events of a tile sorted in LDS at a time (default: what fits two
         workgroups per CU); bucket_mem_frac: share of free device memory the event
         staging may use; tile: threads per workgroup of the step kernels (default:
         1024 for groups of at least 65536 neurons, else 256); npt: neurons per thread
-        (default 1); gather_pairs: (delay, spike)
+        (default 1); overlap: run the gathers concurrently with the group updates when
+        every delay is at least one step (default True); gather_pairs: (delay, spike)
         pairs per gather unit (default: about one register window of events).  These
         shape the launches only: results do not depend on them."""
         self.objs = []
@@ -487,6 +488,7 @@ class Network:
         self.tile = tile
         self.npt = npt
         self.gather_pairs = gather_pairs
+        self.overlap = bool(overlap)
         self.t_step = 0
         self._built = False
         for o in objs:
@@ -558,6 +560,12 @@ class Network:
                 self._build_spikegen(g)
         for s in paths:
             self._build_gather(s)
+        # with every delay at least one step, the gathers of step s read only spikes of earlier
+        # steps: they run concurrently with the group updates of step s
+        self.concurrent_gathers = bool(paths) and all(s.csr["dmin"] >= 1 for s in paths) and self.overlap
+        if self.concurrent_gathers:
+            for s in paths:
+                _lib.check(L.hb_plan_set_lane(self._plan, s.launch_index, 1), "hb_plan_set_lane")
         self._built = True
 
     def _alloc_staging(self, g, incoming):
@@ -600,16 +608,17 @@ class Network:
         del indeg_t, indeg_n
         free, _ = torch.cuda.mem_get_info()
         budget = int(free * self.bucket_mem_frac)
-        g.stg_cap = int(max(1, min(total, 2**32 - 1, (budget // 2) // (8 + vbytes))))
-        g.rows_cap = int(max(1, min(rows, 2**31 - 1, (budget // 4) // (4 * (g.n_tiles + 1)))))
+        # three rotating buffers (one written by this step's gathers, one folded, one cleared)
+        g.stg_cap = int(max(1, min(total, 2**32 - 1, (budget // 2) // (3 * (8 + vbytes)))))
+        g.rows_cap = int(max(1, min(rows, 2**31 - 1, (budget // 4) // (3 * 4 * (g.n_tiles + 1)))))
         big = need_target > g.lds_cap // 2
         g.tile_cap = int(0 if not big else min(need_tile, (budget // 4) // (4 * g.n_tiles)))
-        g.stg_key = torch.empty(g.stg_cap, dtype=torch.int64, device="cuda")
-        g.stg_val = torch.empty(g.stg_cap, dtype=torch_dtype(g.val_t), device="cuda")
-        g.run_tab = torch.zeros(g.rows_cap * (g.n_tiles + 1), dtype=torch.int32, device="cuda")
+        g.stg_key = torch.empty(3 * g.stg_cap, dtype=torch.int64, device="cuda")
+        g.stg_val = torch.empty(3 * g.stg_cap, dtype=torch_dtype(g.val_t), device="cuda")
+        g.run_tab = torch.zeros(3 * g.rows_cap * (g.n_tiles + 1), dtype=torch.int32, device="cuda")
         g.stg_ord = torch.empty(max(1, g.n_tiles * g.tile_cap), dtype=torch.int32, device="cuda")
-        g.stg_top = torch.zeros(2, dtype=torch.int64, device="cuda")
-        g.run_rows = torch.zeros(2, dtype=torch.int32, device="cuda")
+        g.stg_top = torch.zeros(3, dtype=torch.int64, device="cuda")
+        g.run_rows = torch.zeros(3, dtype=torch.int32, device="cuda")
 
     def _value_type(self, s):
         from .codegen import Renderer
@@ -721,7 +730,8 @@ class Network:
         if g.incoming:
             v.update(stg_key=g.stg_key.data_ptr(), stg_val=g.stg_val.data_ptr(), stg_ord=g.stg_ord.data_ptr(),
                      stg_top=g.stg_top.data_ptr(), run_tab=g.run_tab.data_ptr(), run_rows=g.run_rows.data_ptr(),
-                     rows_cap=g.rows_cap, n_tiles=g.n_tiles, tile_cap=g.tile_cap)
+                     rows_cap=g.rows_cap, n_tiles=g.n_tiles, tile_cap=g.tile_cap, stg_stride=g.stg_cap,
+                     tab_stride=g.rows_cap * (g.n_tiles + 1))
         if getattr(g, "spiked", None) is not None:
             v["spiked"] = g.spiked.data_ptr()
         if not flush and g.smon:
@@ -784,7 +794,8 @@ class Network:
              "src0": s.src_range[0], "src1": s.src_range[1], "seg_ptr": s.csr["seg_ptr"].data_ptr(),
              "post": s.csr["post"].data_ptr(), "rows_cap": g.rows_cap, "stg_key": g.stg_key.data_ptr(),
              "stg_val": g.stg_val.data_ptr(), "stg_top": g.stg_top.data_ptr(), "stg_cap": g.stg_cap,
-             "run_tab": g.run_tab.data_ptr(), "run_rows": g.run_rows.data_ptr(), "status": self.status.data_ptr()}
+             "run_tab": g.run_tab.data_ptr(), "run_rows": g.run_rows.data_ptr(), "status": self.status.data_ptr(),
+             "stg_stride": g.stg_cap, "tab_stride": g.rows_cap * (g.n_tiles + 1)}
         for n in s.syn_vars:
             v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
         for n, (val, ct) in sc.items():

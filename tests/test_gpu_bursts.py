@@ -52,3 +52,16 @@ def test_burst_trace_bitexact(case, shape):
     print(f"burst case, launch shape {shape}", flush=True)
     got = device_trace(pre, post, w, d, **shape)
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("shape", [{}, {"tile": 64, "bucket_cap": 128, "gather_pairs": 32}, {"overlap": False}],
+                         ids=["default", "tile64-cap128-pairs32", "no-overlap"])
+def test_burst_trace_bitexact_with_concurrent_gathers(case, shape):
+    """Delays 1-21 steps: the gathers of a step run concurrently with the group
+    update of that step (triple-buffered event staging)."""
+    pre, post, w, d, _ = case
+    d1 = d + 1
+    exp = expected_trace(pre, post, w, d1)
+    print(f"burst case with delays >= 1, launch shape {shape}", flush=True)
+    got = device_trace(pre, post, w, d1, **shape)
+    assert np.array_equal(got, exp)
