@@ -411,39 +411,8 @@ __device__ __forceinline__ float hb_exprelf(float x) {
     if (x > 717.0f) return __builtin_inff();
     return expm1f(x) / x;
 }
-// Sorts one target's run of events by key: insertion sort for short runs, heapsort
-// (O(m log m), in place) for the long runs of synchronous bursts.
-template <typename VT>
-__device__ void hb_sort_kv(unsigned long long* k, VT* v, int n) {
-    if (n <= 24) {
-        for (int a = 1; a < n; a++) {
-            const unsigned long long kk = k[a]; const VT vv = v[a]; int b = a;
-            while (b > 0 && k[b - 1] > kk) { k[b] = k[b - 1]; v[b] = v[b - 1]; b--; }
-            k[b] = kk; v[b] = vv;
-        }
-        return;
-    }
-    for (int start = n / 2 - 1; start >= 0; start--) {
-        int r = start;
-        while (2 * r + 1 < n) {
-            int c = 2 * r + 1;
-            if (c + 1 < n && k[c] < k[c + 1]) c++;
-            if (k[r] < k[c]) { unsigned long long tk = k[r]; k[r] = k[c]; k[c] = tk; VT tv = v[r]; v[r] = v[c]; v[c] = tv; r = c; }
-            else break;
-        }
-    }
-    for (int end = n - 1; end > 0; end--) {
-        { unsigned long long tk = k[0]; k[0] = k[end]; k[end] = tk; VT tv = v[0]; v[0] = v[end]; v[end] = tv; }
-        int r = 0;
-        while (2 * r + 1 < end) {
-            int c = 2 * r + 1;
-            if (c + 1 < end && k[c] < k[c + 1]) c++;
-            if (k[r] < k[c]) { unsigned long long tk = k[r]; k[r] = k[c]; k[c] = tk; VT tv = v[r]; v[r] = v[c]; v[c] = tv; r = c; }
-            else break;
-        }
-    }
-}
-// Same ordering for a run of record indices (global path), comparing keys[idx].
+// Sorts a run of record indices by keys[idx] (one target's events, in global memory):
+// insertion sort for short runs, heapsort (O(m log m), in place) for long ones.
 __device__ void hb_sort_idx(unsigned int* ix, const unsigned long long* keys, int n) {
     if (n <= 24) {
         for (int a = 1; a < n; a++) {
@@ -473,7 +442,9 @@ __device__ void hb_sort_idx(unsigned int* ix, const unsigned long long* keys, in
         }
     }
 }
-// exclusive scan of one int per thread over a workgroup of HB_T threads (HB_T/64 waves)
+// exclusive scan of one int per thread over a workgroup of HB_T threads (HB_T/64 waves):
+// scan within each wave, wave totals through LDS, each thread adds the totals of the
+// waves before its own (sh: HB_T/64 ints, 16-byte aligned)
 __device__ __forceinline__ int hb_block_exscan(int v, int* sh, int* total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int x = v;
@@ -481,16 +452,21 @@ __device__ __forceinline__ int hb_block_exscan(int v, int* sh, int* total) {
     for (int o = 1; o < 64; o <<= 1) { int y = __shfl_up(x, o, 64); if (lane >= o) x += y; }
     if (lane == 63) sh[wid] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int w = 0; w < HB_T / 64; w++) { int t = sh[w]; sh[w] = acc; acc += t; }
-        sh[HB_T / 64] = acc;
+    int pre = 0, tot = 0;
+#if HB_T % 256 == 0
+    #pragma unroll
+    for (int w = 0; w < HB_T / 64; w += 4) {
+        const int4 t = *(const int4*)(sh + w);
+        pre += (w < wid ? t.x : 0) + (w + 1 < wid ? t.y : 0) + (w + 2 < wid ? t.z : 0) + (w + 3 < wid ? t.w : 0);
+        tot += t.x + t.y + t.z + t.w;
     }
+#else
+    #pragma unroll
+    for (int w = 0; w < HB_T / 64; w++) { const int t = sh[w]; pre += w < wid ? t : 0; tot += t; }
+#endif
+    *total = tot;
     __syncthreads();
-    int r = x - v + sh[wid];
-    *total = sh[HB_T / 64];
-    __syncthreads();
-    return r;
+    return x - v + pre;
 }
 """
 

@@ -110,8 +110,6 @@ def _rename(lines, names):
 
 
 FOLD_EVENTS_PER_THREAD = 8    # fold: events a thread keeps in registers
-RANK_SORT_MAX = 32            # fold: targets with at most this many events are ordered by rank counting
-XCD_TILES = True              # step kernels: consecutive tiles on one XCD
 GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per window
 LDS_BYTES = 163840            # LDS of one CU
 
@@ -123,8 +121,8 @@ def fold_capacity(T, val_t, npt=1):
     vbytes = 8 if val_t in ("double", "long long") else 4
     per_wg = LDS_BYTES // max(1, 1024 // T)
     xw = 2 * T * FOLD_EVENTS_PER_THREAD
-    cap = (per_wg - 8 * T * npt - 16 * T - xw - 1024) // (8 + vbytes)
-    # at most T*U: the expansion buffer holds a pass's order
+    cap = (per_wg - 8 * T * npt - xw - 1024) // (8 + vbytes)
+    # at most T*U: the order buffer s_x holds a pass's order
     return int(max(128, min(T * FOLD_EVENTS_PER_THREAD, cap - cap % 64)))
 
 
@@ -132,120 +130,72 @@ def _fold_source(T, cap, vt, flush, npt=1):
     """Fold of the events staged by the previous step's gathers, for a tile of
     T*npt targets (thread x owns targets q*T + x, q < npt).
 
-    Phase A reads this tile's slice of every run and counts events per target in
-    LDS.  Runs are taken T at a time (one row of the run table per thread); their
-    events are numbered in chunks of T*U, and each row writes its index over its
-    events of the chunk in LDS (so an event finds its run with one LDS read).
-    The first T*U events of the tile stay in registers (U per thread).  Phase B
-    computes each target's first event (block scan) and assigns the targets to
-    passes: all of them at once when the tile's events fit in LDS, else by the
-    position of their first event in windows of cap/2 (a pass then covers at most
-    cap events).  In each pass the events of its targets are scattered into LDS,
-    and the owner of each target sorts its run by key and applies it to the
-    target's registers.  A target with more than cap/2 events in one step (an
-    in-degree above cap/2) is sorted through an index list in global memory.
-    When a tile has more than T*U events, the passes read them again."""
+    The tile's events are contiguous in its region of the staging buffer (in the
+    order the gathers' atomics gave them; the keys fix the delivery order).
+    Phase A reads them, the first T*U into registers (U per thread, coalesced),
+    and counts events per target in LDS.  Phase B computes each target's first
+    event (block scan) and assigns the targets to passes: all of them at once
+    when the tile's events fit in LDS, else by the position of their first event
+    in windows of cap/2 (a pass then covers at most cap events).  In each pass
+    the events of its targets are scattered into LDS and ordered by key, and the
+    owner of each target applies them to the target's registers.  A target with
+    more than cap/2 events in one step (an in-degree above cap/2) is sorted
+    through an index list in global memory.  Events beyond the registers are
+    read again by each pass."""
     nw = T // 64
     U = FOLD_EVENTS_PER_THREAD
     XW = T * U
     W = cap // 2
     TN = T * npt
     k = [
-        "    // ---- events delivered by propagate(step - 1): this tile's slice of every run ----",
+        "    // ---- events delivered by propagate(step - 1): this tile's region of the staging ----",
         "    // staging buffers rotate over three steps: step s's gathers write buffer s % 3 (possibly",
-        "    // while this kernel runs), this kernel reads buffer (s - 1) % 3 and clears (s + 1) % 3",
+        "    // while this kernel runs), this kernel reads buffer (s - 1) % 3 and clears the tile's",
+        "    // counter of buffer (s + 1) % 3",
         "    const int hb_nxt = (int)((step + 1) % 3), hb_prv = (int)((step + 2) % 3);",
         "    (void)hb_nxt;",
-        "    const unsigned long long* hb_sk = p.stg_key + (size_t)hb_prv * p.stg_stride;",
-        f"    const {vt}* hb_sv = p.stg_val + (size_t)hb_prv * p.stg_stride;",
-        "    const unsigned int* hb_tab = p.run_tab + (size_t)hb_prv * p.tab_stride;",
+        "    const unsigned int hb_rbase = p.tile_off[hb_tile];",
+        "    const unsigned long long* hb_sk = p.stg_key + (size_t)hb_prv * p.stg_stride + hb_rbase;",
+        f"    const {vt}* hb_sv = p.stg_val + (size_t)hb_prv * p.stg_stride + hb_rbase;",
         f"    __shared__ unsigned long long s_key[{cap}];",
         f"    __shared__ {vt} s_val[{cap}];",
         f"    __shared__ unsigned short s_x[{XW}];",
-        f"    __shared__ unsigned int s_lo[{2 * T}];",
-        f"    __shared__ int s_ro[{2 * T}];",
         f"    __shared__ int s_beg[{TN}];",
         f"    __shared__ int s_fill[{TN}];",
-        f"    __shared__ int s_sh[{nw + 1}];",
-        "    __shared__ int s_R;",
-        "    const size_t hb_rs = (size_t)p.n_tiles + 1;",
-        "    if (threadIdx.x == 0) {",
-        "        const int r = p.run_rows[hb_prv];",
-        "        s_R = r < p.rows_cap ? r : p.rows_cap;",
+        f"    __shared__ __align__(16) int s_sh[{nw + 1}];",
+        "    __shared__ int s_cmax;",
+        "    // events of the tile: the gathers' count, clamped to the region (a count beyond it",
+        "    // was reported by the gather as an overflow)",
+        "    int hb_tot;",
+        "    {",
+        "        const unsigned int c_ = p.tile_cnt[(size_t)hb_prv * p.n_tiles + hb_tile];",
+        "        const unsigned int r_ = p.tile_off[hb_tile + 1] - hb_rbase;",
+        "        const unsigned int n_ = c_ < r_ ? c_ : r_;",
+        "        hb_tot = (int)(n_ < 0x7fffffffu ? n_ : 0x7fffffffu);",
+        "    }",
     ]
     if not flush:
-        k += ["        // the counters of this step's gathers (read by the next step's fold)",
-              "        if (blockIdx.x == 0) { p.run_rows[hb_nxt] = 0; p.stg_top[hb_nxt] = 0ull; }"]
+        k += ["    // the tile's counter of the next step's gathers",
+              "    if (threadIdx.x == 0) p.tile_cnt[(size_t)hb_nxt * p.n_tiles + hb_tile] = 0u;"]
     k += [
-        "    }",
         f"    for (int t_ = threadIdx.x; t_ < {TN}; t_ += {T}) s_fill[t_] = 0;",
+        "    if (threadIdx.x == 0) s_cmax = 0;",
         "    __syncthreads();",
-        "    const int hb_R = s_R;",
         f"    unsigned long long hb_fk[{U}];",
         f"    {vt} hb_fv[{U}];",
-        f"    unsigned int hb_fq[{U}];",
-        "    unsigned int hb_rlo0, hb_rlen0, hb_rlo1, hb_rlen1;",
-        "    int hb_rex, hb_btot;",
-        "    int hb_tot = 0;",
-        # rows r0 + x and r0 + T + x of thread x (the second only where it exists);
-        # s_lo/s_ro by row in the batch
-        "#define HB_ROWS(r0) { \\",
-        "        const int ra_ = (r0) + (int)threadIdx.x, rb_ = ra_ + " + str(T) + "; \\",
-        "        hb_rlo0 = hb_rlo1 = 0u; hb_rlen0 = hb_rlen1 = 0u; \\",
-        "        if (ra_ < hb_R) { hb_rlo0 = hb_tab[ra_ * hb_rs + hb_tile]; "
-        "hb_rlen0 = hb_tab[ra_ * hb_rs + hb_tile + 1] - hb_rlo0; } \\",
-        "        if (rb_ < hb_R) { hb_rlo1 = hb_tab[rb_ * hb_rs + hb_tile]; "
-        "hb_rlen1 = hb_tab[rb_ * hb_rs + hb_tile + 1] - hb_rlo1; } \\",
-        "        hb_rex = hb_block_exscan((int)(hb_rlen0 + hb_rlen1), s_sh, &hb_btot); \\",
-        "        s_lo[threadIdx.x] = hb_rlo0; s_ro[threadIdx.x] = hb_rex; \\",
-        "        s_lo[threadIdx.x + " + str(T) + "] = hb_rlo1; s_ro[threadIdx.x + " + str(T) + "] = hb_rex + (int)hb_rlen0; \\",
-        "        __syncthreads(); }",
-        # s_x[k] = row (in the batch) of event c0 + k of the batch
-        "#define HB_EXPAND(c0) { \\",
-        "        const int a_ = hb_rex > (c0) ? hb_rex : (c0); \\",
-        "        const int e_ = hb_rex + (int)(hb_rlen0 + hb_rlen1); \\",
-        f"        const int b_ = e_ < (c0) + {XW} ? e_ : (c0) + {XW}; \\",
-        "        const int m_ = hb_rex + (int)hb_rlen0; \\",
-        "        for (int x_ = a_; x_ < b_; x_++) s_x[x_ - (c0)] = (unsigned short)(threadIdx.x + (x_ >= m_ ? " + str(T) + " : 0)); \\",
-        "        __syncthreads(); }",
-        "#define HB_POS(c0, kk) (s_lo[s_x[kk]] + (unsigned int)((c0) + (kk) - s_ro[s_x[kk]]))",
-        "    // phase A: events per target of this tile",
-        f"    for (int r0 = 0; r0 < hb_R; r0 += {2 * T}) {{",
-        "        HB_ROWS(r0)",
-        f"        for (int c0 = 0; c0 < hb_btot; c0 += {XW}) {{",
-        f"            const int n = hb_btot - c0 < {XW} ? hb_btot - c0 : {XW};",
-        "            HB_EXPAND(c0)",
-        "            const int base = hb_tot + c0;",
-        "            // events held in registers (event g in slot g / T of thread g % T):",
-        "            // positions first, then all loads in flight at once, then the counts",
-        "            #pragma unroll",
-        f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
-        f"                const int kk = hb_u * {T} + (int)threadIdx.x - base;",
-        "                if (kk >= 0 && kk < n) hb_fq[hb_u] = HB_POS(c0, kk);",
-        "            }",
-        "            #pragma unroll",
-        f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
-        f"                const int kk = hb_u * {T} + (int)threadIdx.x - base;",
-        "                const bool ok = kk >= 0 && kk < n;",
-        "                const unsigned int q = ok ? hb_fq[hb_u] : 0u;",
-        "                const unsigned long long k_ = hb_sk[q];",
-        f"                const {vt} v_ = hb_sv[q];",
-        "                if (ok) { hb_fk[hb_u] = k_; hb_fv[hb_u] = v_; }",
-        "            }",
-        "            #pragma unroll",
-        f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
-        f"                const int kk = hb_u * {T} + (int)threadIdx.x - base;",
-        f"                if (kk >= 0 && kk < n) atomicAdd(&s_fill[(int)(hb_fk[hb_u] >> {KEY_TGT_SHIFT})], 1);",
-        "            }",
-        "            // events beyond the registers: counted here, read again by the passes",
-        f"            if (base + n > {T * U})",
-        f"                for (int kk = threadIdx.x; kk < n; kk += {T})",
-        f"                    if (base + kk >= {T * U}) atomicAdd(&s_fill[(int)(hb_sk[HB_POS(c0, kk)] >> {KEY_TGT_SHIFT})], 1);",
-        "            __syncthreads();",
-        "        }",
-        "        hb_tot += hb_btot;",
-        "        __syncthreads();",
+        "    // phase A: events per target of this tile; event g in slot g / T of thread g % T",
+        "    #pragma unroll",
+        f"    for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+        f"        const int g = hb_u * {T} + (int)threadIdx.x;",
+        "        if (g < hb_tot) { hb_fk[hb_u] = hb_sk[g]; hb_fv[hb_u] = hb_sv[g]; }",
         "    }",
+        "    #pragma unroll",
+        f"    for (int hb_u = 0; hb_u < {U}; hb_u++)",
+        f"        if (hb_u * {T} + (int)threadIdx.x < hb_tot) atomicAdd(&s_fill[(int)(hb_fk[hb_u] >> {KEY_TGT_SHIFT})], 1);",
+        "    // events beyond the registers: counted here, read again by the passes",
+        f"    for (int g = {XW} + (int)threadIdx.x; g < hb_tot; g += {T})",
+        f"        atomicAdd(&s_fill[(int)(hb_sk[g] >> {KEY_TGT_SHIFT})], 1);",
+        "    __syncthreads();",
         "    // phase B: first event of every target (segments in (thread, q) order); passes",
         f"    int hb_c[{npt}], hb_b[{npt}], hb_myp[{npt}];",
         f"    bool hb_bigt[{npt}];",
@@ -255,30 +205,36 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "    int hb_tot2;",
         "    const int hb_tb = hb_block_exscan(hb_ct, s_sh, &hb_tot2);",
         f"    const bool hb_one = hb_tot <= {cap};",
-        f"    const int hb_np = hb_one ? (hb_tot > 0 ? 1 : 0) : (hb_tot + {W - 1}) / {W};",
         "    bool hb_anyb = false;",
+        "    int hb_cm = 0;",
         "    #pragma unroll",
         f"    for (int q = 0; q < {npt}; q++) {{",
         "        hb_b[q] += hb_tb;",
         f"        hb_bigt[q] = !hb_one && hb_c[q] > {W};",
-        f"        hb_myp[q] = hb_one ? 0 : hb_b[q] / {W};",
         f"        s_beg[q * {T} + threadIdx.x] = hb_bigt[q] ? ~hb_b[q] : hb_b[q];",
         f"        s_fill[q * {T} + threadIdx.x] = 0;",
         "        hb_anyb = hb_anyb || hb_bigt[q];",
+        "        if (!hb_bigt[q] && hb_c[q] > hb_cm) hb_cm = hb_c[q];",
         "    }",
+        "    if (!hb_one && hb_cm > 0) atomicMax(&s_cmax, hb_cm);",
         "    const bool hb_anybig = __syncthreads_or(hb_anyb) != 0;",
-        "    // events held in registers are used as they are unless a target is sorted in global",
-        "    // memory (which needs their positions): then the passes read them again",
-        f"    const bool hb_res = hb_tot <= {T * U} && !hb_anybig;",
+        "    // passes: targets by the window of hb_pw positions their first event falls in; a pass",
+        "    // then spans at most hb_pw + (largest count of a target sorted in LDS) = cap events",
+        f"    const int hb_pw = hb_one ? {cap} : {cap} - s_cmax;",
+        "    const int hb_np = hb_one ? (hb_tot > 0 ? 1 : 0) : (hb_tot + hb_pw - 1) / hb_pw;",
+        "    #pragma unroll",
+        f"    for (int q = 0; q < {npt}; q++) hb_myp[q] = hb_one ? 0 : hb_b[q] / hb_pw;",
         "    // bursts beyond the tile's index list (only when memory capped it) are reported, not folded",
         "    const bool hb_ovf = hb_anybig && (long long)hb_tot > p.tile_cap;",
         "    if (hb_ovf && threadIdx.x == 0) atomicOr(p.status, 4);",
         "    unsigned int* hb_go = p.stg_ord + (size_t)hb_tile * (size_t)p.tile_cap;",
+        "    // event (kk, vv) at position pos of the region: into LDS if its target is in this pass,",
+        "    // into the target's index list if the target is sorted in global memory",
         "#define HB_SCATTER(kk, vv, pos) { \\",
         f"        const int t_ = (int)((kk) >> {KEY_TGT_SHIFT}); \\",
         "        const int bt_ = s_beg[t_]; \\",
         "        if (bt_ >= 0) { \\",
-        f"            if (hb_one || bt_ / {W} == hk_pass) {{ \\",
+        "            if (hb_one || (unsigned int)(bt_ - hk_pb) < (unsigned int)hb_pw) { \\",
         "                const int q_ = bt_ + atomicAdd(&s_fill[t_], 1) - hk_pb; \\",
         "                s_key[q_] = (kk); s_val[q_] = (vv); } \\",
         "        } else if (hk_pass == 0 && !hb_ovf) hb_go[~bt_ + atomicAdd(&s_fill[t_], 1)] = (pos); }",
@@ -290,103 +246,58 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
     """Phase C of the fold (see _fold_source).  `apply_lines` apply (hb_k, hb_v) to
     target q's registers.
 
-    One pass (the tile's events fit in LDS): events are scattered by target and
-    each event finds its rank among its target's (event-parallel rank counting).
-    Several passes: per pass, the owners order their targets by rank counting
-    (or heapsort above RANK_SORT_MAX events).  Targets sorted in global memory
-    come last."""
+    Per pass: the events of the pass's targets are scattered into LDS by target
+    (from registers, then the events beyond them from global memory); each event
+    finds its rank among its target's (event-parallel rank counting, keys are
+    distinct) and the order goes to s_x; the owner of each target applies its
+    events in key order.  With several passes, LDS positions that no event of
+    the pass wrote hold stale keys: an entry counts only if it lies inside the
+    segment of a target of the pass, which the pass wrote in full.  Targets
+    sorted in global memory come last."""
     XW = T * U
-
-    def scatter(ind):
-        return [ind + x for x in [
-            "if (hb_res) {",
-            "    #pragma unroll",
-            f"    for (int hb_u = 0; hb_u < {U}; hb_u++)",
-            f"        if (hb_u * {T} + (int)threadIdx.x < hb_tot) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], 0u)",
-            "} else {",
-            f"    for (int r0 = 0; r0 < hb_R; r0 += {2 * T}) {{",
-            "        HB_ROWS(r0)",
-            f"        for (int c0 = 0; c0 < hb_btot; c0 += {XW}) {{",
-            f"            const int n = hb_btot - c0 < {XW} ? hb_btot - c0 : {XW};",
-            "            HB_EXPAND(c0)",
-            f"            for (int kk = threadIdx.x; kk < n; kk += {T}) {{",
-            "                const unsigned int pos = HB_POS(c0, kk);",
-            "                const unsigned long long k_ = hb_sk[pos];",
-            f"                const {vt} v_ = hb_sv[pos];",
-            "                HB_SCATTER(k_, v_, pos)",
-            "            }",
-            "            __syncthreads();",
-            "        }",
-            "        __syncthreads();",
-            "    }",
-            "}",
-            "__syncthreads();",
-        ]]
-
-    def apply(ind, k_expr, v_expr):
-        return ([ind + "{", f"{ind}    const unsigned long long hb_k = {k_expr};", f"{ind}    const {vt} hb_v = {v_expr};"]
-                + [ind + "    " + x for x in apply_lines] + [ind + "}"])
-
-    k = ["    if (hb_one) {", "        const int hk_pass = 0, hk_pb = 0;", "        (void)hk_pass; (void)hk_pb;"]
-    k += scatter("        ")
-    k += [
-        "        // each event finds its rank among its target's (keys are distinct), one event per",
-        "        // thread at a time; the order goes to s_x",
-        f"        for (int x = threadIdx.x; x < hb_tot; x += {T}) {{",
+    TN = T * npt
+    k = [
+        f"    const int hb_lim = hb_one ? hb_tot : {cap};  // LDS positions a pass may have written",
+        "    for (int hk_pass = 0; hk_pass < hb_np; hk_pass++) {",
+        "        const int hk_pb = hk_pass * hb_pw;",
+        "        #pragma unroll",
+        f"        for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+        f"            const int g = hb_u * {T} + (int)threadIdx.x;",
+        "            if (g < hb_tot) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], (unsigned int)g)",
+        "        }",
+        f"        for (int g = {XW} + (int)threadIdx.x; g < hb_tot; g += {T}) HB_SCATTER(hb_sk[g], hb_sv[g], (unsigned int)g)",
+        "        __syncthreads();",
+        f"        for (int x = threadIdx.x; x < hb_lim; x += {T}) {{",
         "            const unsigned long long kx = s_key[x];",
-        f"            const int t_ = (int)(kx >> {KEY_TGT_SHIFT});",
-        "            const int b_ = s_beg[t_], c_ = s_fill[t_];",
-        "            int r = 0;",
-        "            for (int j = 0; j < c_; j++) r += s_key[b_ + j] < kx ? 1 : 0;",
-        "            s_x[b_ + r] = (unsigned short)(x - b_);",
+        f"            int t_ = (int)(kx >> {KEY_TGT_SHIFT});",
+        f"            t_ = t_ < {TN} ? t_ : {TN - 1};",
+        "            const int bt_ = s_beg[t_], b_ = bt_ - hk_pb, c_ = s_fill[t_], e_ = b_ + c_ - 1;",
+        "            if (hb_one || (bt_ >= 0 && (unsigned int)b_ < (unsigned int)hb_pw && x >= b_ && x <= e_)) {",
+        "                int r = 0;",
+        "                if (c_ > 1)",
+        "                    for (int j = b_; j <= e_; j++) r += s_key[j] < kx ? 1 : 0;",
+        "                s_x[b_ + r] = (unsigned short)(x - b_);",
+        "            }",
         "        }",
         "        __syncthreads();",
+        "        // apply: the owner of each target applies its events in key order (the next",
+        "        // event's index is read while the current one is applied)",
         "        #pragma unroll",
         f"        for (int q = 0; q < {npt}; q++) {{",
-        "            const int o = hb_b[q], c = hb_c[q];",
-        "            for (int r = 0; r < c; r++) {",
-        "                const int a = o + (int)s_x[o + r];",
+        "            if (!hb_bigt[q] && hb_myp[q] == hk_pass) {",
+        "                const int o = hb_b[q] - hk_pb, c = hb_c[q];",
+        "                int a = c > 0 ? o + (int)s_x[o] : 0;",
+        "                for (int r = 0; r < c; r++) {",
+        "                    const unsigned long long hb_k = s_key[a];",
+        f"                    const {vt} hb_v = s_val[a];",
+        "                    a = r + 1 < c ? o + (int)s_x[o + r + 1] : a;",
     ]
-    k += apply("                ", "s_key[a]", "s_val[a]")
+    k += ["                    " + x for x in apply_lines]
     k += [
-        "            }",
-        "        }",
-        "    } else {",
-        "        for (int hk_pass = 0; hk_pass < hb_np; hk_pass++) {",
-        f"            const int hk_pb = hk_pass * {W};",
-    ]
-    k += scatter("            ")
-    k += [
-        "            #pragma unroll",
-        f"            for (int q = 0; q < {npt}; q++) {{",
-        "                if (!hb_bigt[q] && hb_myp[q] == hk_pass && hb_c[q] > 0) {",
-        "                    const int o = hb_b[q] - hk_pb;",
-        "                    const int c = hb_c[q];",
-        f"                    if (c <= {RANK_SORT_MAX}) {{",
-        "                        // the owner ranks its target's events (order to s_x, free here)",
-        "                        for (int a = 0; a < c; a++) {",
-        "                            const unsigned long long ka = s_key[o + a];",
-        "                            int r = 0;",
-        "                            for (int b2 = 0; b2 < c; b2++) r += s_key[o + b2] < ka ? 1 : 0;",
-        "                            s_x[o + r] = (unsigned short)a;",
-        "                        }",
-        "                        for (int r = 0; r < c; r++) {",
-        "                            const int a = o + (int)s_x[o + r];",
-    ]
-    k += apply("                            ", "s_key[a]", "s_val[a]")
-    k += [
-        "                        }",
-        "                    } else {",
-        "                        hb_sort_kv(s_key + o, s_val + o, c);",
-        "                        for (int e = o; e < o + c; e++)",
-    ]
-    k += apply("                            ", "s_key[e]", "s_val[e]")
-    k += [
-        "                    }",
         "                }",
         "            }",
-        "            __syncthreads();",
         "        }",
+        "        __syncthreads();",
         "    }",
         "    #pragma unroll",
         f"    for (int q = 0; q < {npt}; q++) {{",
@@ -394,15 +305,14 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         "            hb_sort_idx(hb_go + hb_b[q], hb_sk, hb_c[q]);",
         "            for (int e = hb_b[q]; e < hb_b[q] + hb_c[q]; e++) {",
         "                const unsigned int r = hb_go[e];",
+        "                const unsigned long long hb_k = hb_sk[r];",
+        f"                const {vt} hb_v = hb_sv[r];",
     ]
-    k += apply("                ", "hb_sk[r]", "hb_sv[r]")
+    k += ["                " + x for x in apply_lines]
     k += [
         "            }",
         "        }",
         "    }",
-        "#undef HB_ROWS",
-        "#undef HB_EXPAND",
-        "#undef HB_POS",
         "#undef HB_SCATTER",
     ]
     return k
@@ -469,14 +379,11 @@ class NeuronKernel:
             ps.ptr("stg_key", "unsigned long long", const=True)
             ps.ptr("stg_val", self.val_t, const=True)
             ps.ptr("stg_ord", "unsigned int")
-            ps.ptr("stg_top", "unsigned long long")
-            ps.ptr("run_tab", "unsigned int", const=True)
-            ps.ptr("run_rows", "int")
-            ps.val("rows_cap", "int")
-            ps.val("n_tiles", "int")
+            ps.ptr("tile_off", "unsigned int", const=True)
+            ps.ptr("tile_cnt", "unsigned int")
+            ps.val("n_tiles", "long long")
             ps.val("tile_cap", "long long")
             ps.val("stg_stride", "unsigned long long")
-            ps.val("tab_stride", "unsigned long long")
         if self.reset_mode == "deferred":
             ps.ptr("spiked", "unsigned char")
         ps.ptr("status", "int")
@@ -510,18 +417,7 @@ class NeuronKernel:
         k.append("    const double t = (double)step * p.dt;")
         k.append("    const double dt = p.dt;")
         k.append("    const int N = p.N;")
-        if XCD_TILES:
-            k += [
-                "    // tiles of consecutive neurons go to one XCD (workgroups are dealt to the 8 XCDs in turn),",
-                "    // so the run-table lines and staging slices neighbouring tiles share stay in one L2",
-                "    int hb_tile;",
-                "    {",
-                "        const int G_ = gridDim.x, q_ = G_ / 8, r_ = G_ % 8, x_ = blockIdx.x % 8, k_ = blockIdx.x / 8;",
-                "        hb_tile = x_ < r_ ? x_ * (q_ + 1) + k_ : r_ * (q_ + 1) + (x_ - r_) * q_ + k_;",
-                "    }",
-            ]
-        else:
-            k.append("    const int hb_tile = blockIdx.x;")
+        k.append("    const int hb_tile = blockIdx.x;")
         k += [
             f"    const int hb_i0 = hb_tile * {TN} + threadIdx.x;  // neuron q*{T} + hb_i0 for q < {NPT}",
         ]
@@ -614,17 +510,16 @@ class NeuronKernel:
         k += [
             "    {",
             "        const int slot = (int)(step % p.ring_slots);",
-            f"        __shared__ int s_csh[{nw + 1}];",
+            f"        __shared__ __align__(16) int s_csh[{nw + 1}];",
             "        __shared__ int s_base;",
             "        int hb_n = 0;",
             "        #pragma unroll",
             f"        for (int q = 0; q < {NPT}; q++) hb_n += hb_condq[q] ? 1 : 0;",
-            "        int hb_ntot;",
-            "        const int hb_off = hb_block_exscan(hb_n, s_csh, &hb_ntot);",
         ]
-        if self.spikemon:
+        if not self.spikemon:
+            k += ["        int hb_ntot;", "        const int hb_off = hb_block_exscan(hb_n, s_csh, &hb_ntot);"]
+        else:
             k += [
-                f"        __shared__ int s_cshm[{nw + 1}];",
                 "        __shared__ unsigned long long s_mbase;",
                 "        int hb_nm = 0;",
                 "        #pragma unroll",
@@ -632,9 +527,22 @@ class NeuronKernel:
                 f"            const int i = hb_i0 + q * {T};",
                 "            hb_nm += (hb_condq[q] && i >= p.mon_start && i < p.mon_stop) ? 1 : 0;",
                 "        }",
-                "        int hb_nmtot;",
-                "        const int hb_offm = hb_block_exscan(hb_nm, s_cshm, &hb_nmtot);",
             ]
+            if TN < 32768:
+                k += [
+                    "        // both counts in one scan (each total is at most the tile size, below 2^15)",
+                    "        int hb_pt;",
+                    "        const int hb_po = hb_block_exscan(hb_n | (hb_nm << 16), s_csh, &hb_pt);",
+                    "        const int hb_off = hb_po & 0xffff, hb_offm = hb_po >> 16;",
+                    "        const int hb_ntot = hb_pt & 0xffff, hb_nmtot = hb_pt >> 16;",
+                ]
+            else:
+                k += [
+                    "        int hb_ntot;",
+                    "        const int hb_off = hb_block_exscan(hb_n, s_csh, &hb_ntot);",
+                    "        int hb_nmtot;",
+                    "        const int hb_offm = hb_block_exscan(hb_nm, s_csh, &hb_nmtot);",
+                ]
         k += [
             "        if (threadIdx.x == 0) {",
             "            s_base = hb_ntot ? atomicAdd(&p.ring_cnt[slot], hb_ntot) : 0;",
@@ -680,13 +588,15 @@ class NeuronKernel:
 # ---------------------------------------------------------------------------
 class GatherKernel:
     """Per-pathway event gather: spikes of step-d -> CSR segments of delay d ->
-    tile-sorted runs in the target group's staging area.
+    the regions of the target tiles in the target group's staging buffer.
 
     Work is split in units of `pairs` (delay, spike) pairs (one per thread of the
     first `pairs` threads).  Their segments are flattened into events, handled in
     windows of GATHER_BLOCK * GATHER_EVENTS_PER_THREAD events held in registers:
-    counted per target tile in LDS, written as one run (two global atomics per
-    window: staging space and run-table row)."""
+    counted per target tile in LDS, space taken in each touched tile's region
+    with one global atomic per tile, and the window's events written there
+    (sorted by tile in LDS first when it fits, so that consecutive threads write
+    consecutive positions)."""
 
     def __init__(self, name, rank, n_seg, dmin, on_pre, syn_arrays, scalars, val_t, float_t, target_var,
                  n_tiles, tile, count_synapses=False, pairs=256):
@@ -725,7 +635,6 @@ class GatherKernel:
         ps.val("ring_slots", "int")
         ps.val("src0", "int")
         ps.val("src1", "int")
-        ps.val("rows_cap", "int")
         ps.ptr("seg_ptr", "unsigned int", const=True)
         ps.ptr("post", "int", const=True)
         for v, ct in self.syn_arrays.items():
@@ -734,12 +643,9 @@ class GatherKernel:
             ps.val(f"c_{n}", ct)
         ps.ptr("stg_key", "unsigned long long")
         ps.ptr("stg_val", self.val_t)
-        ps.ptr("stg_top", "unsigned long long")
-        ps.val("stg_cap", "unsigned long long")
-        ps.ptr("run_tab", "unsigned int")
-        ps.ptr("run_rows", "int")
+        ps.ptr("tile_off", "unsigned int", const=True)
+        ps.ptr("tile_cnt", "unsigned int")
         ps.val("stg_stride", "unsigned long long")
-        ps.val("tab_stride", "unsigned long long")
         ps.ptr("status", "int")
         ps.ptr("events", "unsigned long long")  # per-workgroup delivered-event totals
         if self.count_synapses:
@@ -776,22 +682,24 @@ class GatherKernel:
             f"    __shared__ int s_o[{PC}];",
             f"    __shared__ unsigned short s_sg[{PC}];",
             f"    __shared__ int s_hist[{NT}];",
+            f"    __shared__ unsigned int s_base[{NT}];",
             f"    __shared__ unsigned short s_pid[{WN}];",
         ]
         vb = 8 if VT in ("double", "long long") else 4
-        lds = WN * (8 + vb + 2) + NT * 4 + (NS + 1) * 4 + PC * 10 + 1024
-        self.staged = lds <= LDS_BYTES
+        lds = WN * 2 + NT * 8 + (NS + 1) * 4 + PC * 10 + 1024
+        if lds > LDS_BYTES:
+            raise NotImplementedError("too many target tiles for the gather's LDS")
+        self.staged = lds + WN * (8 + vb) <= LDS_BYTES
         if self.staged:
             k += [f"    __shared__ unsigned long long s_ok[{WN}];", f"    __shared__ {VT} s_ov[{WN}];"]
         k += [
-            f"    __shared__ int s_sh[{GT // 64 + 1}];",
-            "    __shared__ unsigned long long s_base;",
-            "    __shared__ int s_row;",
+            f"    __shared__ __align__(16) int s_sh[{GT // 64 + 1}];",
+            "    __shared__ int s_ovw;  // a run of this window overflows its region",
             f"    for (int s = threadIdx.x; s < {NS}; s += {GT}) {{",
             f"        const long long ps_ = step - {self.dmin} - s;",
             "        s_cum[s + 1] = ps_ >= 0 ? p.ring_cnt[(int)(ps_ % p.ring_slots)] : 0;",
             "    }",
-            "    if (threadIdx.x == 0) s_cum[0] = 0;",
+            "    if (threadIdx.x == 0) { s_cum[0] = 0; s_ovw = 0; }",
             f"    for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
             "    __syncthreads();",
             f"    if (threadIdx.x == 0) for (int s = 1; s <= {NS}; s++) s_cum[s] += s_cum[s - 1];",
@@ -880,78 +788,89 @@ class GatherKernel:
             "            }",
             "            #pragma unroll",
             f"            for (int hb_u = 0; hb_u < {U}; hb_u++) if (hb_gt[hb_u] >= 0) atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
-            "            if (threadIdx.x == 0) {",
-            "                s_base = atomicAdd(p.stg_top + par, (unsigned long long)Ew);",
-            "                s_row = atomicAdd(p.run_rows + par, 1);",
-            "            }",
             "            __syncthreads();",
-            "            const unsigned long long base = s_base;",
-            "            const int row = s_row;",
-            "            const bool ok = base + (unsigned long long)Ew <= p.stg_cap && row < p.rows_cap;",
-            "            if (threadIdx.x == 0 && !ok) atomicOr(p.status, 1);",
-            f"            unsigned int* tab = p.run_tab + (size_t)par * p.tab_stride + (size_t)row * {NT + 1};",
-            "            // run-table row: absolute start of every tile's events (all empty if they do not fit)",
-            "            int carry = 0;",
-            f"            for (int t0 = 0; t0 < {NT}; t0 += {GT}) {{",
-            "                const int t = t0 + (int)threadIdx.x;",
-            f"                const int v = t < {NT} ? s_hist[t] : 0;",
-            "                int tot;",
-            "                const int x = hb_block_exscan(v, s_sh, &tot) + carry;",
-            f"                if (t < {NT}) {{",
-            "                    s_hist[t] = x;",
-            "                    if (row < p.rows_cap) tab[t] = ok ? (unsigned int)(base + (unsigned long long)x) : 0u;",
+            "            // space in the regions of the tiles the window touches: one global atomic per tile;",
+            "            // s_base = region-relative start of the window's run.  A run that does not fit is",
+            "            // written up to the region's end (so the region has no unwritten hole) and reported",
+            f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) {{",
+            "                const unsigned int c = (unsigned int)s_hist[t];",
+            "                unsigned int b = 0u;",
+            "                if (c > 0u) {",
+            f"                    b = atomicAdd(p.tile_cnt + (size_t)par * {NT} + t, c);",
+            "                    if ((unsigned long long)b + c > p.tile_off[t + 1] - p.tile_off[t]) { s_ovw = 1; atomicOr(p.status, 1); }",
             "                }",
-            "                carry += tot;",
+            "                s_base[t] = b;",
             "            }",
-            f"            if (threadIdx.x == 0 && row < p.rows_cap) tab[{NT}] = ok ? (unsigned int)(base + (unsigned long long)Ew) : 0u;",
             "            __syncthreads();",
+            "            const bool ovw = s_ovw != 0;",
+            "#define HB_FITS(t, r) (!ovw || (unsigned long long)(r) < p.tile_off[(t) + 1] - p.tile_off[t])",
         ]
         if self.staged:
             k += [
-                "            // the run sorted by tile in LDS, then written out contiguously (coalesced)",
-                "            if (ok) {",
-                "                #pragma unroll",
-                f"                for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
-                "                    if (hb_gt[hb_u] >= 0) {",
-                "                        const int q = atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
-                "                        s_ok[q] = hb_gk[hb_u];",
-                "                        s_ov[q] = hb_gv[hb_u];",
+                "            // the run sorted by tile in LDS (s_pid = tile of each sorted event), then written",
+                "            // out run by run, consecutive threads on consecutive positions",
+                "            int carry = 0;",
+                f"            for (int t0 = 0; t0 < {NT}; t0 += {GT}) {{",
+                "                const int t = t0 + (int)threadIdx.x;",
+                f"                const int v = t < {NT} ? s_hist[t] : 0;",
+                "                int tot;",
+                "                const int x = hb_block_exscan(v, s_sh, &tot) + carry;",
+                f"                if (t < {NT}) s_hist[t] = x;",
+                "                carry += tot;",
+                "            }",
+                "            __syncthreads();",
+                "            #pragma unroll",
+                f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+                "                if (hb_gt[hb_u] >= 0) {",
+                "                    const int q = atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
+                "                    s_ok[q] = hb_gk[hb_u];",
+                "                    s_ov[q] = hb_gv[hb_u];",
+                "                    s_pid[q] = (unsigned short)hb_gt[hb_u];",
             ]
             if self.count_synapses:
-                k.append("                        p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")
+                k.append("                    p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")
             k += [
-                "                    }",
                 "                }",
                 "            }",
                 "            __syncthreads();",
-                "            if (ok) {",
-                f"                for (int x = threadIdx.x; x < Ew; x += {GT}) {{",
-                "                    hb_sk[base + (unsigned long long)x] = s_ok[x];",
-                "                    hb_sv[base + (unsigned long long)x] = s_ov[x];",
+                "            // s_hist[t] is now the end of tile t's run in the window, s_hist[t - 1] its start",
+                f"            for (int x = threadIdx.x; x < Ew; x += {GT}) {{",
+                "                const int t = s_pid[x];",
+                "                const unsigned long long r = (unsigned long long)s_base[t] + (unsigned int)(x - (t > 0 ? s_hist[t - 1] : 0));",
+                "                if (HB_FITS(t, r)) {",
+                "                    const size_t dst = (size_t)p.tile_off[t] + r;",
+                "                    hb_sk[dst] = s_ok[x];",
+                "                    hb_sv[dst] = s_ov[x];",
                 "                }",
                 "            }",
                 "            __syncthreads();",
             ]
         else:
             k += [
-                "            if (ok) {",
-                "                #pragma unroll",
-                f"                for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
-                "                    if (hb_gt[hb_u] >= 0) {",
-                "                        const unsigned long long dst = base + (unsigned long long)atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
+                f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
+                "            __syncthreads();",
+                "            #pragma unroll",
+                f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+                "                if (hb_gt[hb_u] >= 0) {",
+                "                    const int t = hb_gt[hb_u];",
+                "                    const unsigned long long r = (unsigned long long)s_base[t] + (unsigned int)atomicAdd(&s_hist[t], 1);",
+                "                    if (HB_FITS(t, r)) {",
+                "                        const size_t dst = (size_t)p.tile_off[t] + r;",
                 "                        hb_sk[dst] = hb_gk[hb_u];",
                 "                        hb_sv[dst] = hb_gv[hb_u];",
+                "                    }",
             ]
             if self.count_synapses:
-                k.append("                        p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")
+                k.append("                    p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")
             k += [
-                "                    }",
                 "                }",
                 "            }",
                 "            __syncthreads();",
             ]
         k += [
+            "#undef HB_FITS",
             f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
+            "            if (threadIdx.x == 0) s_ovw = 0;",
             "            __syncthreads();",
             "        }",
             "        if (threadIdx.x == 0) hb_ev += (unsigned long long)E;",

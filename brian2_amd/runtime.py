@@ -571,14 +571,16 @@ class Network:
     def _alloc_staging(self, g, incoming):
         """Event staging of one target group.
 
-        stg_key/stg_val: the events of one step, as tile-sorted runs (one per gather
-        window).  Each synapse delivers at most one event per step, so the number of
-        incoming synapses bounds a step's events; run_tab: one row of n_tiles + 1
-        offsets per window, bounded by the units and windows of a step in which every
-        source spikes; stg_ord: per-tile index lists for targets with more events in
-        one step than half the fold's LDS capacity, bounded by the tile's in-degree.
-        Each bound is honoured unless it exceeds the memory budget, in which case an
-        overflow is reported (status bits 1, 4)."""
+        stg_key/stg_val: three rotating buffers (one written by this step's
+        gathers, one folded, one cleared), each split into one region per target
+        tile (tile_off: region starts, n_tiles + 1 offsets).  Each synapse delivers
+        at most one event per step, so a region sized by the tile's incoming
+        synapses never overflows; when the three buffers would exceed the memory
+        budget, every region is scaled down by the same factor and an overflow is
+        reported (status bit 1) instead of writing out of bounds.  tile_cnt: events
+        of each tile in each buffer.  stg_ord: per-tile index lists for targets with
+        more events in one step than half the fold's LDS capacity, bounded by the
+        tile's in-degree (status bit 4 when capped)."""
         torch = _torch()
         T = g.tile
         g.n_tiles = (g.N + T - 1) // T
@@ -587,7 +589,6 @@ class Network:
                         else fold_capacity(g.block, g.val_t, g.npt))
         window = GATHER_BLOCK * GATHER_EVENTS_PER_THREAD
         total = sum(int(s.csr["n_syn"]) for s in incoming)
-        rows = 0
         for s in incoming:
             nsrc = s.src_range[1] - s.src_range[0]
             pairs_max = nsrc * s.csr["n_seg"]
@@ -596,29 +597,30 @@ class Network:
             pc = int(min(GATHER_BLOCK, max(32, 0.85 * window / max(avg, 1e-9))))
             s.pairs = int(self.gather_pairs or pc)
             s.units_max = -(-pairs_max // s.pairs)
-            rows += s.units_max + -(-int(s.csr["n_syn"]) // window)
-        indeg_t = torch.zeros(g.n_tiles, dtype=torch.int64, device="cuda")
         indeg_n = torch.zeros(g.N, dtype=torch.int64, device="cuda")
         for s in incoming:
             post = s.csr["post"][: s.csr["n_syn"]].long()
             indeg_n += torch.bincount(post, minlength=g.N)
-        indeg_t += torch.nn.functional.pad(indeg_n, (0, g.n_tiles * T - g.N)).view(g.n_tiles, T).sum(1)
-        need_tile = int(indeg_t.max().item()) if total else 0
+        indeg_t = torch.nn.functional.pad(indeg_n, (0, g.n_tiles * T - g.N)).view(g.n_tiles, T).sum(1)
+        indeg_t = indeg_t.cpu().numpy().astype(np.int64)
+        need_tile = int(indeg_t.max()) if total else 0
         need_target = int(indeg_n.max().item()) if total else 0
-        del indeg_t, indeg_n
+        del indeg_n
         free, _ = torch.cuda.mem_get_info()
         budget = int(free * self.bucket_mem_frac)
-        # three rotating buffers (one written by this step's gathers, one folded, one cleared)
-        g.stg_cap = int(max(1, min(total, 2**32 - 1, (budget // 2) // (3 * (8 + vbytes)))))
-        g.rows_cap = int(max(1, min(rows, 2**31 - 1, (budget // 4) // (3 * 4 * (g.n_tiles + 1)))))
         big = need_target > g.lds_cap // 2
         g.tile_cap = int(0 if not big else min(need_tile, (budget // 4) // (4 * g.n_tiles)))
+        stg_budget = budget - 4 * g.n_tiles * g.tile_cap
+        limit = int(min(total, 2**32 - 1, stg_budget // (3 * (8 + vbytes))))
+        region = indeg_t if limit >= total else (indeg_t * limit) // max(total, 1)
+        off = np.zeros(g.n_tiles + 1, np.int64)
+        off[1:] = np.cumsum(region)
+        g.stg_cap = int(max(1, off[-1]))
+        g.tile_off = dev(off.astype(np.uint32).view(np.int32))
+        g.tile_cnt = zeros(3 * g.n_tiles, "int")
         g.stg_key = torch.empty(3 * g.stg_cap, dtype=torch.int64, device="cuda")
         g.stg_val = torch.empty(3 * g.stg_cap, dtype=torch_dtype(g.val_t), device="cuda")
-        g.run_tab = torch.zeros(3 * g.rows_cap * (g.n_tiles + 1), dtype=torch.int32, device="cuda")
         g.stg_ord = torch.empty(max(1, g.n_tiles * g.tile_cap), dtype=torch.int32, device="cuda")
-        g.stg_top = torch.zeros(3, dtype=torch.int64, device="cuda")
-        g.run_rows = torch.zeros(3, dtype=torch.int32, device="cuda")
 
     def _value_type(self, s):
         from .codegen import Renderer
@@ -729,9 +731,8 @@ class Network:
             v[f"{tg}_{n}"] = val
         if g.incoming:
             v.update(stg_key=g.stg_key.data_ptr(), stg_val=g.stg_val.data_ptr(), stg_ord=g.stg_ord.data_ptr(),
-                     stg_top=g.stg_top.data_ptr(), run_tab=g.run_tab.data_ptr(), run_rows=g.run_rows.data_ptr(),
-                     rows_cap=g.rows_cap, n_tiles=g.n_tiles, tile_cap=g.tile_cap, stg_stride=g.stg_cap,
-                     tab_stride=g.rows_cap * (g.n_tiles + 1))
+                     tile_off=g.tile_off.data_ptr(), tile_cnt=g.tile_cnt.data_ptr(), n_tiles=g.n_tiles,
+                     tile_cap=g.tile_cap, stg_stride=g.stg_cap)
         if getattr(g, "spiked", None) is not None:
             v["spiked"] = g.spiked.data_ptr()
         if not flush and g.smon:
@@ -792,10 +793,9 @@ class Network:
         v = {"clock": self.clock.data_ptr(), "step_off": 0, "ring_spk": s.source.ring_spk.data_ptr(),
              "ring_cnt": s.source.ring_cnt.data_ptr(), "ring_stride": s.source.N, "ring_slots": s.source.ring_slots,
              "src0": s.src_range[0], "src1": s.src_range[1], "seg_ptr": s.csr["seg_ptr"].data_ptr(),
-             "post": s.csr["post"].data_ptr(), "rows_cap": g.rows_cap, "stg_key": g.stg_key.data_ptr(),
-             "stg_val": g.stg_val.data_ptr(), "stg_top": g.stg_top.data_ptr(), "stg_cap": g.stg_cap,
-             "run_tab": g.run_tab.data_ptr(), "run_rows": g.run_rows.data_ptr(), "status": self.status.data_ptr(),
-             "stg_stride": g.stg_cap, "tab_stride": g.rows_cap * (g.n_tiles + 1)}
+             "post": s.csr["post"].data_ptr(), "stg_key": g.stg_key.data_ptr(),
+             "stg_val": g.stg_val.data_ptr(), "tile_off": g.tile_off.data_ptr(), "tile_cnt": g.tile_cnt.data_ptr(),
+             "status": self.status.data_ptr(), "stg_stride": g.stg_cap}
         for n in s.syn_vars:
             v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
         for n, (val, ct) in sc.items():
@@ -831,7 +831,7 @@ class Network:
         out = [(self, "clock"), (self, "status")]
         for g in self.groups:
             out += [(g, "ring_spk"), (g, "ring_cnt")]
-            for a in ("spiked", "gen_state", "run_rows", "stg_top"):
+            for a in ("spiked", "gen_state", "tile_cnt"):
                 if getattr(g, a, None) is not None:
                     out.append((g, a))
             m = getattr(g, "smon", None)
@@ -928,7 +928,7 @@ class Network:
                 grid = (g.N + g.tile - 1) // g.tile
                 _lib.check(L.hb_launch(g.flush_fn, grid, g.block, 0, stream, buf, len(buf)), "flush")
                 if getattr(g, "incoming", None):
-                    g.run_rows.zero_()  # the flushed events are not folded again by the next run
+                    g.tile_cnt.zero_()  # the flushed events are not folded again by the next run
         for g in self.groups:
             m = getattr(g, "stmon", None)
             if m is not None and n_steps > 0:
