@@ -606,8 +606,11 @@ class GatherKernel:
             raise NotImplementedError(f"delay range of more than {MAX_SEG} steps")
         if n_tiles > 16384:
             raise NotImplementedError("more than 16384 target tiles")
-        if not 1 <= pairs <= GATHER_BLOCK:
-            raise ValueError("pairs per unit must be in [1, GATHER_BLOCK]")
+        self.block = GATHER_BLOCK
+        self.events_per_thread = GATHER_EVENTS_PER_THREAD
+        self.window = self.block * self.events_per_thread
+        if not 1 <= pairs <= self.block:
+            raise ValueError(f"pairs per unit must be in [1, {self.block}]")
         self.name = name
         self.rank = rank
         self.n_seg = n_seg
@@ -651,8 +654,8 @@ class GatherKernel:
         if self.count_synapses:
             ps.ptr("syn_count", "int")           # per-synapse event counts (CSR order)
         self.struct = ps
-        GT = GATHER_BLOCK
-        U = GATHER_EVENTS_PER_THREAD
+        GT = self.block
+        U = self.events_per_thread
         WN = GT * U
         PC = self.pairs
         NS = self.n_seg
@@ -690,19 +693,29 @@ class GatherKernel:
         if lds > LDS_BYTES:
             raise NotImplementedError("too many target tiles for the gather's LDS")
         self.staged = lds + WN * (8 + vb) <= LDS_BYTES
+        self.lds_bytes = lds + (WN * (8 + vb) if self.staged else 0)
+        # workgroups that share a CU: LDS, and 16 waves of at most 128 VGPRs
+        self.wgs_per_cu = max(1, min(LDS_BYTES // self.lds_bytes, 1024 // GT))
         if self.staged:
             k += [f"    __shared__ unsigned long long s_ok[{WN}];", f"    __shared__ {VT} s_ov[{WN}];"]
         k += [
             f"    __shared__ __align__(16) int s_sh[{GT // 64 + 1}];",
             "    __shared__ int s_ovw;  // a run of this window overflows its region",
-            f"    for (int s = threadIdx.x; s < {NS}; s += {GT}) {{",
-            f"        const long long ps_ = step - {self.dmin} - s;",
-            "        s_cum[s + 1] = ps_ >= 0 ? p.ring_cnt[(int)(ps_ % p.ring_slots)] : 0;",
-            "    }",
+            "    // s_cum[s]: (delay, spike) pairs of the delay segments before s (spikes of step - dmin - s)",
             "    if (threadIdx.x == 0) { s_cum[0] = 0; s_ovw = 0; }",
             f"    for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
-            "    __syncthreads();",
-            f"    if (threadIdx.x == 0) for (int s = 1; s <= {NS}; s++) s_cum[s] += s_cum[s - 1];",
+            "    {",
+            "        int carry = 0;",
+            f"        for (int s0 = 0; s0 < {NS}; s0 += {GT}) {{",
+            "            const int s = s0 + (int)threadIdx.x;",
+            f"            const long long ps_ = step - {self.dmin} - s;",
+            f"            const int n_ = s < {NS} && ps_ >= 0 ? p.ring_cnt[(int)(ps_ % p.ring_slots)] : 0;",
+            "            int tot;",
+            "            const int x = hb_block_exscan(n_, s_sh, &tot) + carry;",
+            f"            if (s < {NS}) s_cum[s + 1] = x + n_;",
+            "            carry += tot;",
+            "        }",
+            "    }",
             "    __syncthreads();",
             f"    const int P = s_cum[{NS}];",
             f"    const int n_units = (P + {PC - 1}) / {PC};",
@@ -790,20 +803,21 @@ class GatherKernel:
             f"            for (int hb_u = 0; hb_u < {U}; hb_u++) if (hb_gt[hb_u] >= 0) atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
             "            __syncthreads();",
             "            // space in the regions of the tiles the window touches: one global atomic per tile;",
-            "            // s_base = region-relative start of the window's run.  A run that does not fit is",
-            "            // written up to the region's end (so the region has no unwritten hole) and reported",
+            "            // s_base = buffer position of the window's run.  A run that does not fit is written",
+            "            // up to the region's end (so the region has no unwritten hole) and reported",
             f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) {{",
             "                const unsigned int c = (unsigned int)s_hist[t];",
-            "                unsigned int b = 0u;",
             "                if (c > 0u) {",
-            f"                    b = atomicAdd(p.tile_cnt + (size_t)par * {NT} + t, c);",
-            "                    if ((unsigned long long)b + c > p.tile_off[t + 1] - p.tile_off[t]) { s_ovw = 1; atomicOr(p.status, 1); }",
+            "                    const unsigned int lo = p.tile_off[t], hi = p.tile_off[t + 1];",
+            f"                    const unsigned int b = atomicAdd(p.tile_cnt + (size_t)par * {NT} + t, c);",
+            "                    if ((unsigned long long)b + c > hi - lo) { s_ovw = 1; atomicOr(p.status, 1); }",
+            "                    s_base[t] = b < hi - lo ? lo + b : hi;  // position in the buffer",
             "                }",
-            "                s_base[t] = b;",
             "            }",
             "            __syncthreads();",
             "            const bool ovw = s_ovw != 0;",
-            "#define HB_FITS(t, r) (!ovw || (unsigned long long)(r) < p.tile_off[(t) + 1] - p.tile_off[t])",
+            "            // an event at buffer position r of tile t fits unless r passes the region's end",
+            "#define HB_FITS(t, r) (!ovw || (r) < (unsigned long long)p.tile_off[(t) + 1])",
         ]
         if self.staged:
             k += [
@@ -838,9 +852,8 @@ class GatherKernel:
                 "                const int t = s_pid[x];",
                 "                const unsigned long long r = (unsigned long long)s_base[t] + (unsigned int)(x - (t > 0 ? s_hist[t - 1] : 0));",
                 "                if (HB_FITS(t, r)) {",
-                "                    const size_t dst = (size_t)p.tile_off[t] + r;",
-                "                    hb_sk[dst] = s_ok[x];",
-                "                    hb_sv[dst] = s_ov[x];",
+                "                    hb_sk[r] = s_ok[x];",
+                "                    hb_sv[r] = s_ov[x];",
                 "                }",
                 "            }",
                 "            __syncthreads();",
@@ -855,9 +868,8 @@ class GatherKernel:
                 "                    const int t = hb_gt[hb_u];",
                 "                    const unsigned long long r = (unsigned long long)s_base[t] + (unsigned int)atomicAdd(&s_hist[t], 1);",
                 "                    if (HB_FITS(t, r)) {",
-                "                        const size_t dst = (size_t)p.tile_off[t] + r;",
-                "                        hb_sk[dst] = hb_gk[hb_u];",
-                "                        hb_sv[dst] = hb_gv[hb_u];",
+                "                        hb_sk[r] = hb_gk[hb_u];",
+                "                        hb_sv[r] = hb_gv[hb_u];",
                 "                    }",
             ]
             if self.count_synapses:

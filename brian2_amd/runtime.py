@@ -18,11 +18,10 @@ import os
 import numpy as np
 
 from . import _lib
+from . import kernels as _kernels
 from .codegen import ctype_of
 from .kernels import (
     FOLD_EVENTS_PER_THREAD,
-    GATHER_BLOCK,
-    GATHER_EVENTS_PER_THREAD,
     CodeSpec,
     GatherKernel,
     NeuronKernel,
@@ -66,6 +65,7 @@ class Device:
         self.device = device
         self.ctx = ctypes.c_void_p()
         _lib.check(L.hb_ctx_create(device, ctypes.byref(self.ctx)), "hb_ctx_create")
+        self.n_cu = int(torch.cuda.get_device_properties(device).multi_processor_count)
         self._modules = {}
 
     @classmethod
@@ -587,14 +587,14 @@ class Network:
         vbytes = 8 if g.val_t in ("double", "long long") else 4
         g.lds_cap = int(min(self.bucket_cap, g.block * FOLD_EVENTS_PER_THREAD) if self.bucket_cap
                         else fold_capacity(g.block, g.val_t, g.npt))
-        window = GATHER_BLOCK * GATHER_EVENTS_PER_THREAD
+        window = _kernels.GATHER_BLOCK * _kernels.GATHER_EVENTS_PER_THREAD
         total = sum(int(s.csr["n_syn"]) for s in incoming)
         for s in incoming:
             nsrc = s.src_range[1] - s.src_range[0]
             pairs_max = nsrc * s.csr["n_seg"]
             # pairs per unit: about one window of events per unit on average
             avg = s.csr["n_syn"] / max(1, pairs_max)
-            pc = int(min(GATHER_BLOCK, max(32, 0.85 * window / max(avg, 1e-9))))
+            pc = int(min(_kernels.GATHER_BLOCK, max(32, 0.85 * window / max(avg, 1e-9))))
             s.pairs = int(self.gather_pairs or pc)
             s.units_max = -(-pairs_max // s.pairs)
         indeg_n = torch.zeros(g.N, dtype=torch.int64, device="cuda")
@@ -801,7 +801,8 @@ class Network:
         for n, (val, ct) in sc.items():
             v[f"c_{n}"] = val
         nsrc = s.src_range[1] - s.src_range[0]
-        grid = int(min(256, max(1, s.units_max)))
+        # enough workgroups to fill every CU (units are taken in turn)
+        grid = int(min(self.device.n_cu * K.wgs_per_cu, max(1, s.units_max)))
         s.events_dev = _torch().zeros(grid, dtype=_torch().int64, device="cuda")
         v["events"] = s.events_dev.data_ptr()
         if s.count_events:
@@ -813,7 +814,7 @@ class Network:
                 setattr(a, name, v[name])
         s.launch_index = len(self._launches)
         s.args, s.grid = a, grid
-        self._add_launch(s.fn, grid, GATHER_BLOCK, a, cls)
+        self._add_launch(s.fn, grid, K.block, a, cls)
 
     def _add_launch(self, fn, grid, block, args, cls, step_field="step_off"):
         off = getattr(cls, step_field).offset
