@@ -442,14 +442,24 @@ __device__ void hb_sort_idx(unsigned int* ix, const unsigned long long* keys, in
         }
     }
 }
+// inclusive scan over the 64 lanes of a wave: shifts within rows of 16 lanes, then the
+// last lane of row 0 added to row 1 and of row 2 to row 3, then that of row 1 to rows 2-3
+// (DPP row_shr and row_bcast; lanes whose source is out of the row contribute 0)
+__device__ __forceinline__ int hb_wave_incl_scan(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return x;
+}
 // exclusive scan of one int per thread over a workgroup of HB_T threads (HB_T/64 waves):
 // scan within each wave, wave totals through LDS, each thread adds the totals of the
 // waves before its own (sh: HB_T/64 ints, 16-byte aligned)
 __device__ __forceinline__ int hb_block_exscan(int v, int* sh, int* total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int x = v;
-    #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) { int y = __shfl_up(x, o, 64); if (lane >= o) x += y; }
+    const int x = hb_wave_incl_scan(v);
     if (lane == 63) sh[wid] = x;
     __syncthreads();
     int pre = 0, tot = 0;

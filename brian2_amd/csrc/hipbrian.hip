@@ -364,7 +364,10 @@ extern "C" int hb_plan_profile(hb_plan* p, void* stream, int64_t n_steps, float*
     HB_HIP(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const size_t nl = p->launches.size();
-    std::vector<hipEvent_t> ev(nl + 1);
+    // steps are enqueued in chunks with an event after every launch, so that the device never
+    // waits for the host inside a chunk and each interval between events is one launch
+    const int64_t chunk = 64;
+    std::vector<hipEvent_t> ev((size_t)chunk * (nl + 1));
     // timing-only events: no system-scope fence (its cache writeback would be timed as part of
     // the kernel before it)
     for (auto& e : ev) HB_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
@@ -372,25 +375,35 @@ extern "C" int hb_plan_profile(hb_plan* p, void* stream, int64_t n_steps, float*
     HB_HIP(hipEventRecord(c->ev_in, (hipStream_t)stream));
     HB_HIP(hipStreamWaitEvent(s, c->ev_in, 0));
     int rc = 0;
-    for (int64_t j = 0; j < n_steps && !rc; j++) {
-        HB_HIP(hipEventRecord(ev[0], s));
-        for (size_t k = 0; k < nl && !rc; k++) {
-            auto& r = p->launches[k];
-            if (r.step_off >= 0) memcpy(r.args.data() + r.step_off, &j, 8);
-            if (r.grid) rc = launch_fn(r.fn, r.grid, r.block, r.shmem, s, r.args.data(), r.args.size());
-            HB_HIP(hipEventRecord(ev[k + 1], s));
+    for (int64_t j0 = 0; j0 < n_steps && !rc; j0 += chunk) {
+        const int64_t nj = n_steps - j0 < chunk ? n_steps - j0 : chunk;
+        for (int64_t jj = 0; jj < nj && !rc; jj++) {
+            const int64_t j = j0 + jj;
+            hipEvent_t* e = ev.data() + (size_t)jj * (nl + 1);
+            HB_HIP(hipEventRecord(e[0], s));
+            for (size_t k = 0; k < nl && !rc; k++) {
+                auto& r = p->launches[k];
+                if (r.step_off >= 0) memcpy(r.args.data() + r.step_off, &j, 8);
+                if (r.grid) rc = launch_fn(r.fn, r.grid, r.block, r.shmem, s, r.args.data(), r.args.size());
+                HB_HIP(hipEventRecord(e[k + 1], s));
+            }
         }
-        HB_HIP(hipEventSynchronize(ev[nl]));
-        for (size_t k = 0; k < nl; k++) {
-            float ms = 0.0f;
-            HB_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-            ms_per_launch[k] += ms;
+        if (rc) break;
+        HB_HIP(hipEventSynchronize(ev[(size_t)(nj - 1) * (nl + 1) + nl]));
+        for (int64_t jj = 0; jj < nj; jj++) {
+            hipEvent_t* e = ev.data() + (size_t)jj * (nl + 1);
+            for (size_t k = 0; k < nl; k++) {
+                float ms = 0.0f;
+                HB_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+                ms_per_launch[k] += ms;
+            }
         }
     }
     if (!rc) {
         hipLaunchKernelGGL(hb_k_clock_advance, dim3(1), dim3(64), 0, s, p->clock, n_steps);
         HB_HIP(hipGetLastError());
     }
+    HB_HIP(hipStreamSynchronize(s));
     for (auto& e : ev) hipEventDestroy(e);
     HB_HIP(hipEventRecord(c->ev_out, s));
     HB_HIP(hipStreamWaitEvent((hipStream_t)stream, c->ev_out, 0));

@@ -114,16 +114,21 @@ GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per w
 LDS_BYTES = 163840            # LDS of one CU
 
 
+def fold_lds_bytes(T, cap, val_t, npt=1):
+    """LDS of a step kernel's fold: key, value and order slot per event, first
+    event and count per target, and a little for scans and flags."""
+    vbytes = 8 if val_t in ("double", "long long") else 4
+    return cap * (8 + vbytes + 2) + 8 * T * npt + 1024
+
+
 def fold_capacity(T, val_t, npt=1):
     """Events of one tile sorted in LDS at a time.  A step kernel needs about 120
     VGPRs, i.e. 16 waves per CU, so max(1, 1024 // T) workgroups of T threads
     share the LDS of a CU; each gets an equal share."""
     vbytes = 8 if val_t in ("double", "long long") else 4
     per_wg = LDS_BYTES // max(1, 1024 // T)
-    xw = 2 * T * FOLD_EVENTS_PER_THREAD
-    cap = (per_wg - 8 * T * npt - xw - 1024) // (8 + vbytes)
-    # at most T*U: the order buffer s_x holds a pass's order
-    return int(max(128, min(T * FOLD_EVENTS_PER_THREAD, cap - cap % 64)))
+    cap = (per_wg - 8 * T * npt - 1024) // (8 + vbytes + 2)
+    return int(max(128, cap - cap % 64))
 
 
 def _fold_source(T, cap, vt, flush, npt=1):
@@ -159,7 +164,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
         f"    const {vt}* hb_sv = p.stg_val + (size_t)hb_prv * p.stg_stride + hb_rbase;",
         f"    __shared__ unsigned long long s_key[{cap}];",
         f"    __shared__ {vt} s_val[{cap}];",
-        f"    __shared__ unsigned short s_x[{XW}];",
+        f"    __shared__ unsigned short s_x[{cap}];",
         f"    __shared__ int s_beg[{TN}];",
         f"    __shared__ int s_fill[{TN}];",
         f"    __shared__ __align__(16) int s_sh[{nw + 1}];",
@@ -177,18 +182,20 @@ def _fold_source(T, cap, vt, flush, npt=1):
     if not flush:
         k += ["    // the tile's counter of the next step's gathers",
               "    if (threadIdx.x == 0) p.tile_cnt[(size_t)hb_nxt * p.n_tiles + hb_tile] = 0u;"]
-    k += [
+    pro = k
+    k = [
         f"    for (int t_ = threadIdx.x; t_ < {TN}; t_ += {T}) s_fill[t_] = 0;",
         "    if (threadIdx.x == 0) s_cmax = 0;",
-        "    __syncthreads();",
         f"    unsigned long long hb_fk[{U}];",
         f"    {vt} hb_fv[{U}];",
-        "    // phase A: events per target of this tile; event g in slot g / T of thread g % T",
+        "    // phase A: events per target of this tile; event g in slot g / T of thread g % T.  The",
+        "    // loads are issued before the barrier, so that they and the state loads complete together",
         "    #pragma unroll",
         f"    for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
         f"        const int g = hb_u * {T} + (int)threadIdx.x;",
         "        if (g < hb_tot) { hb_fk[hb_u] = hb_sk[g]; hb_fv[hb_u] = hb_sv[g]; }",
         "    }",
+        "    __syncthreads();",
         "    #pragma unroll",
         f"    for (int hb_u = 0; hb_u < {U}; hb_u++)",
         f"        if (hb_u * {T} + (int)threadIdx.x < hb_tot) atomicAdd(&s_fill[(int)(hb_fk[hb_u] >> {KEY_TGT_SHIFT})], 1);",
@@ -239,7 +246,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "                s_key[q_] = (kk); s_val[q_] = (vv); } \\",
         "        } else if (hk_pass == 0 && !hb_ovf) hb_go[~bt_ + atomicAdd(&s_fill[t_], 1)] = (pos); }",
     ]
-    return k, U, W
+    return pro, k, U, W
 
 
 def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
@@ -336,8 +343,8 @@ class NeuronKernel:
         self.fold = fold or []        # [(rank, var, op)]
         self.val_t = val_t
         self.cap = fold_capacity(tile, val_t, npt) if not lds_cap else int(lds_cap)
-        if self.cap < 128 or self.cap % 2 or self.cap > tile * FOLD_EVENTS_PER_THREAD:
-            raise ValueError(f"fold capacity must be an even number of events in [128, {tile * FOLD_EVENTS_PER_THREAD}]")
+        if self.cap < 128 or self.cap % 2 or fold_lds_bytes(tile, self.cap, val_t, npt) > LDS_BYTES:
+            raise ValueError("fold capacity must be an even number of events, at least 128, whose LDS fits a CU")
         self.spikemon = spikemon      # (start, stop) or None
         self.statemon = statemon      # [vars] or None
         self.statemon_when = statemon_when
@@ -422,6 +429,9 @@ class NeuronKernel:
             f"    const int hb_i0 = hb_tile * {TN} + threadIdx.x;  // neuron q*{T} + hb_i0 for q < {NPT}",
         ]
         k.append("    (void)t; (void)dt;")
+        fold = _fold_source(T, self.cap, self.val_t, self.flush, NPT) if self.fold else None
+        if fold:
+            k += fold[0]  # the tile's event count first: phase A's loads depend on it, not on the state
         # state of this thread's neurons in registers (the fold updates them)
         for v, ct in self.arrays.items():
             k.append(f"    {'bool' if ct == 'bool' else ct} hb_a_{v}[{NPT}];")
@@ -438,7 +448,7 @@ class NeuronKernel:
         k.append("        hb_condq[q] = false;")
         k.append("    }")
         if self.fold:
-            fk, U, W = _fold_source(T, self.cap, self.val_t, self.flush, NPT)
+            fk, U, W = fold[1], fold[2], fold[3]
             k += fk
             apply = ["switch ((int)((hb_k >> 44) & 15)) {"]
             for rank, var, op in self.fold:

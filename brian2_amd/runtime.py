@@ -21,7 +21,6 @@ from . import _lib
 from . import kernels as _kernels
 from .codegen import ctype_of
 from .kernels import (
-    FOLD_EVENTS_PER_THREAD,
     CodeSpec,
     GatherKernel,
     NeuronKernel,
@@ -585,7 +584,7 @@ class Network:
         T = g.tile
         g.n_tiles = (g.N + T - 1) // T
         vbytes = 8 if g.val_t in ("double", "long long") else 4
-        g.lds_cap = int(min(self.bucket_cap, g.block * FOLD_EVENTS_PER_THREAD) if self.bucket_cap
+        g.lds_cap = int(min(self.bucket_cap, fold_capacity(g.block, g.val_t, g.npt)) if self.bucket_cap
                         else fold_capacity(g.block, g.val_t, g.npt))
         window = _kernels.GATHER_BLOCK * _kernels.GATHER_EVENTS_PER_THREAD
         total = sum(int(s.csr["n_syn"]) for s in incoming)
