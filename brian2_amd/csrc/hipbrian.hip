@@ -47,11 +47,14 @@ extern "C" int hb_last_error(char* buf, size_t n) {
 // ---------------------------------------------------------------------------
 // context, memory
 // ---------------------------------------------------------------------------
+#define HB_LANES 4  // lane 0: the plan's stream; lanes 1..HB_LANES-1: side streams
+
 struct hb_ctx {
     int device;
-    hipStream_t stream;  // default priority; graph replay happens here
-    hipStream_t side;    // default priority; launches of lane 1 (concurrent with lane 0 within a step)
-    hipEvent_t ev_in, ev_out, ev_fork, ev_join;
+    hipStream_t stream;                // default priority; graph replay happens here
+    hipStream_t side[HB_LANES - 1];    // default priority; launches of lanes 1.. (concurrent within a step)
+    hipEvent_t ev_in, ev_out, ev_fork;
+    hipEvent_t ev_join[HB_LANES - 1];
 };
 
 extern "C" int hb_device_count(int* n) {
@@ -64,11 +67,13 @@ extern "C" int hb_ctx_create(int device, hb_ctx** out) {
     std::unique_ptr<hb_ctx> c(new hb_ctx());
     c->device = device;
     HB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HB_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    for (int l = 0; l < HB_LANES - 1; l++) {
+        HB_HIP(hipStreamCreateWithFlags(&c->side[l], hipStreamNonBlocking));
+        HB_HIP(hipEventCreateWithFlags(&c->ev_join[l], hipEventDisableTiming));
+    }
     HB_HIP(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
     HB_HIP(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
     HB_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    HB_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     *out = c.release();
     return 0;
 }
@@ -76,13 +81,15 @@ extern "C" int hb_ctx_create(int device, hb_ctx** out) {
 extern "C" int hb_ctx_destroy(hb_ctx* c) {
     if (!c) return 0;
     HB_HIP(hipStreamSynchronize(c->stream));
-    HB_HIP(hipStreamSynchronize(c->side));
+    for (int l = 0; l < HB_LANES - 1; l++) HB_HIP(hipStreamSynchronize(c->side[l]));
     HB_HIP(hipEventDestroy(c->ev_in));
     HB_HIP(hipEventDestroy(c->ev_out));
     HB_HIP(hipEventDestroy(c->ev_fork));
-    HB_HIP(hipEventDestroy(c->ev_join));
+    for (int l = 0; l < HB_LANES - 1; l++) {
+        HB_HIP(hipEventDestroy(c->ev_join[l]));
+        HB_HIP(hipStreamDestroy(c->side[l]));
+    }
     HB_HIP(hipStreamDestroy(c->stream));
-    HB_HIP(hipStreamDestroy(c->side));
     delete c;
     return 0;
 }
@@ -235,7 +242,7 @@ struct hb_launch_rec {
     unsigned grid, block, shmem;
     std::vector<char> args;
     long step_off;
-    int lane = 0;  // 0: in order on the plan's stream; 1: concurrently with lane 0 within the step
+    int lane = 0;  // 0: in order on the plan's stream; l > 0: on side stream l, concurrently within the step
 };
 
 struct hb_plan {
@@ -281,7 +288,7 @@ extern "C" int hb_plan_add(hb_plan* p, void* fn, unsigned grid, unsigned block, 
 
 extern "C" int hb_plan_set_lane(hb_plan* p, int index, int lane) {
     if (index < 0 || index >= (int)p->launches.size()) return set_err("hb_plan_set_lane: bad index");
-    if (lane != 0 && lane != 1) return set_err("hb_plan_set_lane: lane must be 0 or 1");
+    if (lane < 0 || lane >= HB_LANES) return set_err("hb_plan_set_lane: lane must be in [0, 4)");
     p->launches[index].lane = lane;
     return plan_drop_graph(p);
 }
@@ -295,27 +302,31 @@ extern "C" int hb_plan_set_args(hb_plan* p, int index, const void* args, size_t 
 }
 
 // Enqueues `k` steps (launch records with patched step offsets) and the clock advance.
-// Within a step, lane-1 launches run on the side stream, forked after the previous
-// step and joined before the next one; lane-0 launches run in order on `s`.
+// Within a step, the launches of lane l > 0 run in order on side stream l, forked after
+// the previous step and joined before the next one; lane-0 launches run in order on `s`.
 static int plan_enqueue(hb_plan* p, hipStream_t s, int64_t k) {
+    bool used[HB_LANES] = {false};
     bool lanes = false;
-    for (auto& r : p->launches) lanes = lanes || (r.lane == 1 && r.grid);
+    for (auto& r : p->launches)
+        if (r.grid && r.lane > 0) { used[r.lane] = true; lanes = true; }
     hb_ctx* c = p->ctx;
     for (int64_t j = 0; j < k; j++) {
         if (lanes) {
             HB_HIP(hipEventRecord(c->ev_fork, s));
-            HB_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            for (int l = 1; l < HB_LANES; l++)
+                if (used[l]) HB_HIP(hipStreamWaitEvent(c->side[l - 1], c->ev_fork, 0));
         }
         for (auto& r : p->launches) {
             if (r.grid == 0) continue;
             if (r.step_off >= 0) memcpy(r.args.data() + r.step_off, &j, 8);
-            hipStream_t ls = (lanes && r.lane == 1) ? c->side : s;
+            hipStream_t ls = r.lane > 0 ? c->side[r.lane - 1] : s;
             int rc = launch_fn(r.fn, r.grid, r.block, r.shmem, ls, r.args.data(), r.args.size());
             if (rc) return rc;
         }
-        if (lanes) {
-            HB_HIP(hipEventRecord(c->ev_join, c->side));
-            HB_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
+        for (int l = 1; l < HB_LANES; l++) {
+            if (!used[l]) continue;
+            HB_HIP(hipEventRecord(c->ev_join[l - 1], c->side[l - 1]));
+            HB_HIP(hipStreamWaitEvent(s, c->ev_join[l - 1], 0));
         }
     }
     hipLaunchKernelGGL(hb_k_clock_advance, dim3(1), dim3(64), 0, s, p->clock, k);

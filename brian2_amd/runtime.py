@@ -560,11 +560,12 @@ class Network:
         for s in paths:
             self._build_gather(s)
         # with every delay at least one step, the gathers of step s read only spikes of earlier
-        # steps: they run concurrently with the group updates of step s
+        # steps: they run concurrently with the group updates of step s and with each other
+        # (three side lanes, taken in turn)
         self.concurrent_gathers = bool(paths) and all(s.csr["dmin"] >= 1 for s in paths) and self.overlap
         if self.concurrent_gathers:
-            for s in paths:
-                _lib.check(L.hb_plan_set_lane(self._plan, s.launch_index, 1), "hb_plan_set_lane")
+            for i, s in enumerate(paths):
+                _lib.check(L.hb_plan_set_lane(self._plan, s.launch_index, 1 + i % 3), "hb_plan_set_lane")
         self._built = True
 
     def _alloc_staging(self, g, incoming):

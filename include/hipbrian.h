@@ -78,11 +78,12 @@ int hb_plan_create(hb_ctx* ctx, int64_t* clock, hb_plan** out);
 int hb_plan_add(hb_plan* plan, void* fn, unsigned grid, unsigned block, unsigned shmem, const void* args,
                 size_t argsize, long step_arg_offset);
 /* Puts launch `index` in lane 0 (default: in plan order on the plan's stream) or
- * lane 1 (a second default-priority stream, forked at the start of each step and
- * joined at its end): lane-1 launches of a step may run concurrently with its
- * lane-0 launches.  For pathways whose delays are all at least one step, the
- * propagation of step s reads only spikes of earlier steps, so it can overlap
- * the group update of step s (Network.run's per-step order, core/network.py:
+ * lane 1, 2 or 3 (further default-priority streams, forked at the start of each
+ * step and joined at its end): the launches of different lanes of a step may run
+ * concurrently, those of one lane run in plan order.  For pathways whose delays
+ * are all at least one step, the propagation of step s reads only spikes of
+ * earlier steps, so it can overlap the group update of step s and the other
+ * pathways' propagation (Network.run's per-step order, core/network.py:
  * 1194-1250, is kept wherever results depend on it). */
 int hb_plan_set_lane(hb_plan* plan, int index, int lane);
 /* Replaces the argument buffer of launch `index` (drops a captured graph). */
