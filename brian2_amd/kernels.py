@@ -13,19 +13,18 @@ per pathway -> reset -> clock.advance.  On the device one step is
                               have delay d, d in [dmin, dmax] (the spike queue as
                               a ring of spike lists indexed by integer delay)
 
-Event staging.  The gather works on chunks of 256 (delay, spike) pairs.  A
-workgroup flattens the chunk's synapse segments into events, counts them per
-target tile in LDS, reserves space for the whole chunk with one global atomic,
-and writes the events sorted by tile (a "run") plus one row of per-tile offsets
-into a run table.  Only two global atomics are issued per chunk; no event is
-appended with an atomic.
+Event staging.  The gather works on units of (delay, spike) pairs.  A workgroup
+flattens a unit's synapse segments into events and handles them in windows held
+in registers: counted per target tile in LDS, space taken in each touched tile's
+region of the staging buffer with one global atomic per tile, and the window's
+events written there, sorted by tile.  No event is appended with its own atomic.
 
-Fold.  The neuron kernel of a tile reads its slice of every run, counting-sorts
-the events by target in LDS, and every target sorts its own events by key before
-applying them one by one.  The key (pathway rank, dmax - delay, CSR position) is
-ordered exactly like (pathway in schedule order, push step, source index,
-synapse index), i.e. like the cython runtime's sequential left fold over the
-spike queue (brian2/codegen/runtime/cython_rt/templates/synapses.pyx:42-50).
+Fold.  The neuron kernel of a tile reads its region (contiguous), counting-sorts
+the events by target in LDS, ranks every event among its target's by key and
+applies them one by one in that order.  The key (pathway rank, dmax - delay, CSR
+position) is ordered exactly like (pathway in schedule order, push step, source
+index, synapse index), i.e. like the cython runtime's sequential left fold over
+the spike queue (brian2/codegen/runtime/cython_rt/templates/synapses.pyx:42-50).
 No floating-point atomics are used, so the sums are bit-identical to Brian's.
 """
 import re

@@ -3,8 +3,9 @@ fold, against the sequential fp64 fold in the cython runtime's delivery order
 (itself checked against the oracle on the CPU).
 
 Launch shapes force multi-pass folds (LDS capacity 128 events), per-target
-global sorting (hub targets above half that capacity), more runs per step than
-threads per tile (the fold re-reads the runs), and tiles of 64 and 1024.
+global sorting (hub targets above half that capacity), more events per tile than
+the fold's registers hold (the passes read the rest again), gathers of a few
+pairs per unit (many windows per step) and tiles of 64 and 1024.
 """
 import numpy as np
 import pytest
