@@ -454,6 +454,16 @@ __device__ __forceinline__ int hb_wave_incl_scan(int x) {
     x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
     return x;
 }
+// offset of this thread's v items in a workgroup-wide allocation counted in *ctr (LDS):
+// a prefix within the wave, and one LDS atomic per wave for the wave's base (waves in
+// arbitrary order).  *ctr must be zero before; after a barrier it holds the total.
+__device__ __forceinline__ int hb_block_alloc(int v, int* ctr) {
+    const int x = hb_wave_incl_scan(v);
+    const int tot = __builtin_amdgcn_readlane(x, 63);
+    int base = 0;
+    if ((threadIdx.x & 63) == 63 && tot != 0) base = atomicAdd(ctr, tot);
+    return __builtin_amdgcn_readlane(base, 63) + x - v;
+}
 // exclusive scan of one int per thread over a workgroup of HB_T threads (HB_T/64 waves):
 // scan within each wave, wave totals through LDS, each thread adds the totals of the
 // waves before its own (sh: HB_T/64 ints, 16-byte aligned)

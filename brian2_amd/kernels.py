@@ -146,7 +146,6 @@ def _fold_source(T, cap, vt, flush, npt=1):
     more than cap/2 events in one step (an in-degree above cap/2) is sorted
     through an index list in global memory.  Events beyond the registers are
     read again by each pass."""
-    nw = T // 64
     U = FOLD_EVENTS_PER_THREAD
     XW = T * U
     W = cap // 2
@@ -166,8 +165,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
         f"    __shared__ unsigned short s_x[{cap}];",
         f"    __shared__ int s_beg[{TN}];",
         f"    __shared__ int s_fill[{TN}];",
-        f"    __shared__ __align__(16) int s_sh[{nw + 1}];",
-        "    __shared__ int s_cmax;",
+        "    __shared__ int s_cmax, s_alloc;",
         "    // events of the tile: the gathers' count, clamped to the region (a count beyond it",
         "    // was reported by the gather as an overflow)",
         "    int hb_tot;",
@@ -184,7 +182,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
     pro = k
     k = [
         f"    for (int t_ = threadIdx.x; t_ < {TN}; t_ += {T}) s_fill[t_] = 0;",
-        "    if (threadIdx.x == 0) s_cmax = 0;",
+        "    if (threadIdx.x == 0) { s_cmax = 0; s_alloc = 0; }",
         f"    unsigned long long hb_fk[{U}];",
         f"    {vt} hb_fv[{U}];",
         "    // phase A: events per target of this tile; event g in slot g / T of thread g % T.  The",
@@ -208,8 +206,8 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "    int hb_ct = 0;",
         "    #pragma unroll",
         f"    for (int q = 0; q < {npt}; q++) {{ hb_c[q] = s_fill[q * {T} + threadIdx.x]; hb_b[q] = hb_ct; hb_ct += hb_c[q]; }}",
-        "    int hb_tot2;",
-        "    const int hb_tb = hb_block_exscan(hb_ct, s_sh, &hb_tot2);",
+        "    // segments in LDS: an allocation, waves in any order (the passes need no particular order)",
+        "    const int hb_tb = hb_block_alloc(hb_ct, &s_alloc);",
         f"    const bool hb_one = hb_tot <= {cap};",
         "    bool hb_anyb = false;",
         "    int hb_cm = 0;",
@@ -414,7 +412,6 @@ class NeuronKernel:
     def _source(self, ps):
         T, NPT = self.T, self.npt
         TN = T * NPT
-        nw = T // 64
         k = [source_header(T), ps.c_decl()]
         kname = f"{'flush' if self.flush else 'step'}_{self.name}"
         self.kernel_name = kname
@@ -428,6 +425,9 @@ class NeuronKernel:
             f"    const int hb_i0 = hb_tile * {TN} + threadIdx.x;  // neuron q*{T} + hb_i0 for q < {NPT}",
         ]
         k.append("    (void)t; (void)dt;")
+        if not self.flush:
+            # spike counts of the workgroup (cleared here, before the first barrier)
+            k += ["    __shared__ int s_nsp, s_nspm;", "    if (threadIdx.x == 0) { s_nsp = 0; s_nspm = 0; }"]
         fold = _fold_source(T, self.cap, self.val_t, self.flush, NPT) if self.fold else None
         if fold:
             k += fold[0]  # the tile's event count first: phase A's loads depend on it, not on the state
@@ -515,19 +515,18 @@ class NeuronKernel:
         if self.flush:
             k.append("}")
             return "\n".join(k) + "\n"
-        # ---- spikes of this step -> ring slot (workgroup prefix + one atomic per workgroup)
+        # ---- spikes of this step -> ring slot (workgroup allocation + one atomic per workgroup)
         k += [
             "    {",
             "        const int slot = (int)(step % p.ring_slots);",
-            f"        __shared__ __align__(16) int s_csh[{nw + 1}];",
             "        __shared__ int s_base;",
             "        int hb_n = 0;",
             "        #pragma unroll",
             f"        for (int q = 0; q < {NPT}; q++) hb_n += hb_condq[q] ? 1 : 0;",
         ]
-        if not self.spikemon:
-            k += ["        int hb_ntot;", "        const int hb_off = hb_block_exscan(hb_n, s_csh, &hb_ntot);"]
-        else:
+        if not self.fold:
+            k.append("        __syncthreads();  // s_nsp cleared")
+        if self.spikemon:
             k += [
                 "        __shared__ unsigned long long s_mbase;",
                 "        int hb_nm = 0;",
@@ -537,27 +536,26 @@ class NeuronKernel:
                 "            hb_nm += (hb_condq[q] && i >= p.mon_start && i < p.mon_stop) ? 1 : 0;",
                 "        }",
             ]
-            if TN < 32768:
-                k += [
-                    "        // both counts in one scan (each total is at most the tile size, below 2^15)",
-                    "        int hb_pt;",
-                    "        const int hb_po = hb_block_exscan(hb_n | (hb_nm << 16), s_csh, &hb_pt);",
-                    "        const int hb_off = hb_po & 0xffff, hb_offm = hb_po >> 16;",
-                    "        const int hb_ntot = hb_pt & 0xffff, hb_nmtot = hb_pt >> 16;",
-                ]
-            else:
-                k += [
-                    "        int hb_ntot;",
-                    "        const int hb_off = hb_block_exscan(hb_n, s_csh, &hb_ntot);",
-                    "        int hb_nmtot;",
-                    "        const int hb_offm = hb_block_exscan(hb_nm, s_csh, &hb_nmtot);",
-                ]
+        packed = self.spikemon and TN < 32768
+        if packed:
+            k += [
+                "        // both counts in one allocation (each total is at most the tile size, below 2^15)",
+                "        const int hb_po = hb_block_alloc(hb_n | (hb_nm << 16), &s_nsp);",
+                "        const int hb_off = hb_po & 0xffff, hb_offm = hb_po >> 16;",
+            ]
+        else:
+            k.append("        const int hb_off = hb_block_alloc(hb_n, &s_nsp);")
+            if self.spikemon:
+                k.append("        const int hb_offm = hb_block_alloc(hb_nm, &s_nspm);")
         k += [
+            "        __syncthreads();",
             "        if (threadIdx.x == 0) {",
+            "            const int hb_ntot = s_nsp & 0xffff;" if packed else "            const int hb_ntot = s_nsp;",
             "            s_base = hb_ntot ? atomicAdd(&p.ring_cnt[slot], hb_ntot) : 0;",
         ]
         if self.spikemon:
             k += [
+                "            const int hb_nmtot = s_nsp >> 16;" if packed else "            const int hb_nmtot = s_nspm;",
                 "            s_mbase = hb_nmtot ? atomicAdd(p.mon_n, (unsigned long long)hb_nmtot) : 0ull;",
                 "            if (hb_nmtot && s_mbase + (unsigned long long)hb_nmtot > (unsigned long long)p.mon_cap) "
                 "atomicOr(p.status, 2);",
