@@ -727,26 +727,37 @@ class GatherKernel:
             f"    const int P = s_cum[{NS}];",
             f"    const int n_units = (P + {PC - 1}) / {PC};",
             "    unsigned long long hb_ev = 0ull;",
+            "    // one (delay, spike) pair per thread of a unit: its delay segment and source (read",
+            "    // from the ring), then its synapse segment; a unit's pairs are read while the",
+            "    // previous unit is processed (the source at its start, the segment before its last",
+            "    // window is written out)",
+            "#define HB_PAIR_SRC(c_, seg_, src_) { \\",
+            f"        const int q_ = (c_) * {PC} + (int)threadIdx.x; \\",
+            "        src_ = -1; seg_ = 0; \\",
+            f"        if ((int)threadIdx.x < {PC} && q_ < P) {{ \\",
+            f"            int lo = 0, hi = {NS}; \\",
+            "            while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (s_cum[m] <= q_) lo = m; else hi = m; } \\",
+            "            seg_ = lo; \\",
+            f"            const int slot = (int)((step - {self.dmin} - lo) % p.ring_slots); \\",
+            "            src_ = p.ring_spk[(size_t)slot * p.ring_stride + (q_ - s_cum[lo])]; } }",
+            "#define HB_PAIR_SEG(seg_, src_, a_, len_) { \\",
+            "        a_ = 0u; len_ = 0u; \\",
+            "        if (src_ >= p.src0 && src_ < p.src1) { \\",
+            f"            const size_t row = (size_t)(src_ - p.src0) * {NS} + (seg_); \\",
+            "            a_ = p.seg_ptr[row]; len_ = p.seg_ptr[row + 1] - a_; } }",
+            "    int n_seg = 0, n_src = -1;",
+            "    unsigned int n_a = 0u, n_len = 0u;",
+            "    if ((int)blockIdx.x < n_units) {",
+            "        HB_PAIR_SRC((int)blockIdx.x, n_seg, n_src)",
+            "        HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
+            "    }",
             "    for (int c = blockIdx.x; c < n_units; c += gridDim.x) {",
             f"        const int q0 = c * {PC};",
             f"        const int nq = P - q0 < {PC} ? P - q0 : {PC};",
-            "        // one (delay, spike) pair per thread: its synapse segment",
-            "        unsigned int a = 0, len = 0;",
-            "        int seg = 0;",
-            "        if ((int)threadIdx.x < nq) {",
-            "            const int qa = q0 + (int)threadIdx.x;",
-            f"            int lo = 0, hi = {NS};",
-            "            while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (s_cum[m] <= qa) lo = m; else hi = m; }",
-            "            seg = lo;",
-            f"            const long long sstep = step - {self.dmin} - seg;",
-            "            const int slot = (int)(sstep % p.ring_slots);",
-            "            const int src = p.ring_spk[(size_t)slot * p.ring_stride + (qa - s_cum[seg])];",
-            "            if (src >= p.src0 && src < p.src1) {",
-            f"                const size_t row = (size_t)(src - p.src0) * {NS} + seg;",
-            "                a = p.seg_ptr[row];",
-            "                len = p.seg_ptr[row + 1] - a;",
-            "            }",
-            "        }",
+            "        const unsigned int a = n_a, len = n_len;",
+            "        const int seg = n_seg;",
+            "        const bool hb_more = c + (int)gridDim.x < n_units;",
+            "        if (hb_more) HB_PAIR_SRC(c + (int)gridDim.x, n_seg, n_src)",
             "        int E;",
             "        const int ex = hb_block_exscan((int)len, s_sh, &E);",
             f"        if ((int)threadIdx.x < {PC}) {{ s_a[threadIdx.x] = a; s_o[threadIdx.x] = ex; s_sg[threadIdx.x] = (unsigned short)seg; }}",
@@ -854,6 +865,7 @@ class GatherKernel:
                 "                }",
                 "            }",
                 "            __syncthreads();",
+                f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
                 "            // s_hist[t] is now the end of tile t's run in the window, s_hist[t - 1] its start",
                 f"            for (int x = threadIdx.x; x < Ew; x += {GT}) {{",
                 "                const int t = s_pid[x];",
@@ -868,6 +880,7 @@ class GatherKernel:
         else:
             k += [
                 f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
+                f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
                 "            __syncthreads();",
                 "            #pragma unroll",
                 f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
@@ -892,8 +905,11 @@ class GatherKernel:
             "            if (threadIdx.x == 0) s_ovw = 0;",
             "            __syncthreads();",
             "        }",
+            "        if (E == 0 && hb_more) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
             "        if (threadIdx.x == 0) hb_ev += (unsigned long long)E;",
             "    }",
+            "#undef HB_PAIR_SRC",
+            "#undef HB_PAIR_SEG",
             "    if (threadIdx.x == 0) p.events[blockIdx.x] += hb_ev;",
             "}",
         ]
