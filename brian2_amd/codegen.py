@@ -411,13 +411,30 @@ __device__ __forceinline__ float hb_exprelf(float x) {
     if (x > 717.0f) return __builtin_inff();
     return expm1f(x) / x;
 }
-// Sorts a run of record indices by keys[idx] (one target's events, in global memory):
-// insertion sort for short runs, heapsort (O(m log m), in place) for long ones.
-__device__ void hb_sort_idx(unsigned int* ix, const unsigned long long* keys, int n) {
+// Staged event records: the 64-bit order key and the pathway value's bits, one record per
+// event (key and value of an event share cache lines when written and read).
+struct hb_rec4 { unsigned int k0, k1, v; };                                   // 4-byte values, 12 B
+struct __attribute__((aligned(16))) hb_rec8 { unsigned long long k, v; };     // 8-byte values, 16 B
+__device__ __forceinline__ unsigned long long hb_rkey(const hb_rec4& r) {
+    return ((unsigned long long)r.k1 << 32) | r.k0;
+}
+__device__ __forceinline__ unsigned long long hb_rkey(const hb_rec8& r) { return r.k; }
+template <typename V> __device__ __forceinline__ V hb_rval(const hb_rec4& r) { return __builtin_bit_cast(V, r.v); }
+template <typename V> __device__ __forceinline__ V hb_rval(const hb_rec8& r) { return __builtin_bit_cast(V, r.v); }
+template <typename V> __device__ __forceinline__ void hb_rset(hb_rec4& r, unsigned long long k, V v) {
+    r.k0 = (unsigned int)k; r.k1 = (unsigned int)(k >> 32); r.v = __builtin_bit_cast(unsigned int, v);
+}
+template <typename V> __device__ __forceinline__ void hb_rset(hb_rec8& r, unsigned long long k, V v) {
+    r.k = k; r.v = __builtin_bit_cast(unsigned long long, v);
+}
+// Sorts a run of record indices by the records' keys (one target's events, in global
+// memory): insertion sort for short runs, heapsort (O(m log m), in place) for long ones.
+template <typename R>
+__device__ void hb_sort_idx(unsigned int* ix, const R* recs, int n) {
     if (n <= 24) {
         for (int a = 1; a < n; a++) {
-            const unsigned int xx = ix[a]; const unsigned long long kk = keys[xx]; int b = a;
-            while (b > 0 && keys[ix[b - 1]] > kk) { ix[b] = ix[b - 1]; b--; }
+            const unsigned int xx = ix[a]; const unsigned long long kk = hb_rkey(recs[xx]); int b = a;
+            while (b > 0 && hb_rkey(recs[ix[b - 1]]) > kk) { ix[b] = ix[b - 1]; b--; }
             ix[b] = xx;
         }
         return;
@@ -426,8 +443,8 @@ __device__ void hb_sort_idx(unsigned int* ix, const unsigned long long* keys, in
         int r = start;
         while (2 * r + 1 < n) {
             int c = 2 * r + 1;
-            if (c + 1 < n && keys[ix[c]] < keys[ix[c + 1]]) c++;
-            if (keys[ix[r]] < keys[ix[c]]) { unsigned int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
+            if (c + 1 < n && hb_rkey(recs[ix[c]]) < hb_rkey(recs[ix[c + 1]])) c++;
+            if (hb_rkey(recs[ix[r]]) < hb_rkey(recs[ix[c]])) { unsigned int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
             else break;
         }
     }
@@ -436,8 +453,8 @@ __device__ void hb_sort_idx(unsigned int* ix, const unsigned long long* keys, in
         int r = 0;
         while (2 * r + 1 < end) {
             int c = 2 * r + 1;
-            if (c + 1 < end && keys[ix[c]] < keys[ix[c + 1]]) c++;
-            if (keys[ix[r]] < keys[ix[c]]) { unsigned int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
+            if (c + 1 < end && hb_rkey(recs[ix[c]]) < hb_rkey(recs[ix[c + 1]])) c++;
+            if (hb_rkey(recs[ix[r]]) < hb_rkey(recs[ix[c]])) { unsigned int t = ix[r]; ix[r] = ix[c]; ix[c] = t; r = c; }
             else break;
         }
     }

@@ -113,6 +113,15 @@ GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per w
 LDS_BYTES = 163840            # LDS of one CU
 
 
+def rec_type(val_t):
+    """Staged event record of a value type: 12 bytes for 4-byte values, 16 for 8-byte."""
+    return "hb_rec8" if val_t in ("double", "long long") else "hb_rec4"
+
+
+def rec_words(val_t):
+    return 4 if val_t in ("double", "long long") else 3
+
+
 def fold_lds_bytes(T, cap, val_t, npt=1):
     """LDS of a step kernel's fold: key, value and order slot per event, first
     event and count per target, and a little for scans and flags."""
@@ -158,8 +167,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "    const int hb_nxt = (int)((step + 1) % 3), hb_prv = (int)((step + 2) % 3);",
         "    (void)hb_nxt;",
         "    const unsigned int hb_rbase = p.tile_off[hb_tile];",
-        "    const unsigned long long* hb_sk = p.stg_key + (size_t)hb_prv * p.stg_stride + hb_rbase;",
-        f"    const {vt}* hb_sv = p.stg_val + (size_t)hb_prv * p.stg_stride + hb_rbase;",
+        f"    const {rec_type(vt)}* hb_st = (const {rec_type(vt)}*)p.stg_rec + (size_t)hb_prv * p.stg_stride + hb_rbase;",
         f"    __shared__ unsigned long long s_key[{cap}];",
         f"    __shared__ {vt} s_val[{cap}];",
         f"    __shared__ unsigned short s_x[{cap}];",
@@ -190,7 +198,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "    #pragma unroll",
         f"    for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
         f"        const int g = hb_u * {T} + (int)threadIdx.x;",
-        "        if (g < hb_tot) { hb_fk[hb_u] = hb_sk[g]; hb_fv[hb_u] = hb_sv[g]; }",
+        f"        if (g < hb_tot) {{ const auto r_ = hb_st[g]; hb_fk[hb_u] = hb_rkey(r_); hb_fv[hb_u] = hb_rval<{vt}>(r_); }}",
         "    }",
         "    __syncthreads();",
         "    #pragma unroll",
@@ -198,7 +206,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
         f"        if (hb_u * {T} + (int)threadIdx.x < hb_tot) atomicAdd(&s_fill[(int)(hb_fk[hb_u] >> {KEY_TGT_SHIFT})], 1);",
         "    // events beyond the registers: counted here, read again by the passes",
         f"    for (int g = {XW} + (int)threadIdx.x; g < hb_tot; g += {T})",
-        f"        atomicAdd(&s_fill[(int)(hb_sk[g] >> {KEY_TGT_SHIFT})], 1);",
+        f"        atomicAdd(&s_fill[(int)(hb_rkey(hb_st[g]) >> {KEY_TGT_SHIFT})], 1);",
         "    __syncthreads();",
         "    // phase B: first event of every target (segments in (thread, q) order); passes",
         f"    int hb_c[{npt}], hb_b[{npt}], hb_myp[{npt}];",
@@ -269,7 +277,10 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         f"            const int g = hb_u * {T} + (int)threadIdx.x;",
         "            if (g < hb_tot) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], (unsigned int)g)",
         "        }",
-        f"        for (int g = {XW} + (int)threadIdx.x; g < hb_tot; g += {T}) HB_SCATTER(hb_sk[g], hb_sv[g], (unsigned int)g)",
+        f"        for (int g = {XW} + (int)threadIdx.x; g < hb_tot; g += {T}) {{",
+        "            const auto r_ = hb_st[g];",
+        f"            HB_SCATTER(hb_rkey(r_), hb_rval<{vt}>(r_), (unsigned int)g)",
+        "        }",
         "        __syncthreads();",
         f"        for (int x = threadIdx.x; x < hb_lim; x += {T}) {{",
         "            const unsigned long long kx = s_key[x];",
@@ -306,11 +317,11 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         "    #pragma unroll",
         f"    for (int q = 0; q < {npt}; q++) {{",
         "        if (hb_bigt[q] && !hb_ovf) {",
-        "            hb_sort_idx(hb_go + hb_b[q], hb_sk, hb_c[q]);",
+        "            hb_sort_idx(hb_go + hb_b[q], hb_st, hb_c[q]);",
         "            for (int e = hb_b[q]; e < hb_b[q] + hb_c[q]; e++) {",
         "                const unsigned int r = hb_go[e];",
-        "                const unsigned long long hb_k = hb_sk[r];",
-        f"                const {vt} hb_v = hb_sv[r];",
+        "                const unsigned long long hb_k = hb_rkey(hb_st[r]);",
+        f"                const {vt} hb_v = hb_rval<{vt}>(hb_st[r]);",
     ]
     k += ["                " + x for x in apply_lines]
     k += [
@@ -380,8 +391,7 @@ class NeuronKernel:
         ps.ptr("ring_cnt", "int")
         ps.val("ring_stride", "long long")
         if self.fold:
-            ps.ptr("stg_key", "unsigned long long", const=True)
-            ps.ptr("stg_val", self.val_t, const=True)
+            ps.ptr("stg_rec", "unsigned int", const=True)
             ps.ptr("stg_ord", "unsigned int")
             ps.ptr("tile_off", "unsigned int", const=True)
             ps.ptr("tile_cnt", "unsigned int")
@@ -651,8 +661,7 @@ class GatherKernel:
             ps.ptr(f"s_{v}", ct, const=True)
         for n, (_, ct) in sorted(self.scalars.items()):
             ps.val(f"c_{n}", ct)
-        ps.ptr("stg_key", "unsigned long long")
-        ps.ptr("stg_val", self.val_t)
+        ps.ptr("stg_rec", "unsigned int")
         ps.ptr("tile_off", "unsigned int", const=True)
         ps.ptr("tile_cnt", "unsigned int")
         ps.val("stg_stride", "unsigned long long")
@@ -685,8 +694,7 @@ class GatherKernel:
         k += [
             "    const long long step = p.clock[0] + p.step_off;",
             "    const int par = (int)(step % 3);  // staging buffer of this step",
-            "    unsigned long long* hb_sk = p.stg_key + (size_t)par * p.stg_stride;",
-            f"    {VT}* hb_sv = p.stg_val + (size_t)par * p.stg_stride;",
+            f"    {rec_type(VT)}* hb_st = ({rec_type(VT)}*)p.stg_rec + (size_t)par * p.stg_stride;",
             f"    __shared__ int s_cum[{NS + 1}];",
             f"    __shared__ unsigned int s_a[{PC}];",
             f"    __shared__ int s_o[{PC}];",
@@ -871,8 +879,7 @@ class GatherKernel:
                 "                const int t = s_pid[x];",
                 "                const unsigned long long r = (unsigned long long)s_base[t] + (unsigned int)(x - (t > 0 ? s_hist[t - 1] : 0));",
                 "                if (HB_FITS(t, r)) {",
-                "                    hb_sk[r] = s_ok[x];",
-                "                    hb_sv[r] = s_ov[x];",
+                "                    hb_rset(hb_st[r], s_ok[x], s_ov[x]);",
                 "                }",
                 "            }",
                 "            __syncthreads();",
@@ -888,8 +895,7 @@ class GatherKernel:
                 "                    const int t = hb_gt[hb_u];",
                 "                    const unsigned long long r = (unsigned long long)s_base[t] + (unsigned int)atomicAdd(&s_hist[t], 1);",
                 "                    if (HB_FITS(t, r)) {",
-                "                        hb_sk[r] = hb_gk[hb_u];",
-                "                        hb_sv[r] = hb_gv[hb_u];",
+                "                        hb_rset(hb_st[r], hb_gk[hb_u], hb_gv[hb_u]);",
                 "                    }",
             ]
             if self.count_synapses:

@@ -25,6 +25,7 @@ from .kernels import (
     GatherKernel,
     NeuronKernel,
     fold_capacity,
+    rec_words,
     identifiers,
     scalar_values,
 )
@@ -571,9 +572,9 @@ class Network:
     def _alloc_staging(self, g, incoming):
         """Event staging of one target group.
 
-        stg_key/stg_val: three rotating buffers (one written by this step's
-        gathers, one folded, one cleared), each split into one region per target
-        tile (tile_off: region starts, n_tiles + 1 offsets).  Each synapse delivers
+        stg_rec: three rotating buffers of event records (order key and value; one
+        written by this step's gathers, one folded, one cleared), each split into
+        one region per target tile (tile_off: region starts, n_tiles + 1 offsets).  Each synapse delivers
         at most one event per step, so a region sized by the tile's incoming
         synapses never overflows; when the three buffers would exceed the memory
         budget, every region is scaled down by the same factor and an overflow is
@@ -618,8 +619,7 @@ class Network:
         g.stg_cap = int(max(1, off[-1]))
         g.tile_off = dev(off.astype(np.uint32).view(np.int32))
         g.tile_cnt = zeros(3 * g.n_tiles, "int")
-        g.stg_key = torch.empty(3 * g.stg_cap, dtype=torch.int64, device="cuda")
-        g.stg_val = torch.empty(3 * g.stg_cap, dtype=torch_dtype(g.val_t), device="cuda")
+        g.stg_rec = torch.empty(3 * g.stg_cap * rec_words(g.val_t), dtype=torch.int32, device="cuda")
         g.stg_ord = torch.empty(max(1, g.n_tiles * g.tile_cap), dtype=torch.int32, device="cuda")
 
     def _value_type(self, s):
@@ -730,7 +730,7 @@ class Network:
         for (tg, n), (val, ct) in g.kernel.scalars.items():
             v[f"{tg}_{n}"] = val
         if g.incoming:
-            v.update(stg_key=g.stg_key.data_ptr(), stg_val=g.stg_val.data_ptr(), stg_ord=g.stg_ord.data_ptr(),
+            v.update(stg_rec=g.stg_rec.data_ptr(), stg_ord=g.stg_ord.data_ptr(),
                      tile_off=g.tile_off.data_ptr(), tile_cnt=g.tile_cnt.data_ptr(), n_tiles=g.n_tiles,
                      tile_cap=g.tile_cap, stg_stride=g.stg_cap)
         if getattr(g, "spiked", None) is not None:
@@ -793,8 +793,8 @@ class Network:
         v = {"clock": self.clock.data_ptr(), "step_off": 0, "ring_spk": s.source.ring_spk.data_ptr(),
              "ring_cnt": s.source.ring_cnt.data_ptr(), "ring_stride": s.source.N, "ring_slots": s.source.ring_slots,
              "src0": s.src_range[0], "src1": s.src_range[1], "seg_ptr": s.csr["seg_ptr"].data_ptr(),
-             "post": s.csr["post"].data_ptr(), "stg_key": g.stg_key.data_ptr(),
-             "stg_val": g.stg_val.data_ptr(), "tile_off": g.tile_off.data_ptr(), "tile_cnt": g.tile_cnt.data_ptr(),
+             "post": s.csr["post"].data_ptr(), "stg_rec": g.stg_rec.data_ptr(),
+             "tile_off": g.tile_off.data_ptr(), "tile_cnt": g.tile_cnt.data_ptr(),
              "status": self.status.data_ptr(), "stg_stride": g.stg_cap}
         for n in s.syn_vars:
             v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
