@@ -113,6 +113,25 @@ GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per w
 LDS_BYTES = 163840            # LDS of one CU
 
 
+def synapse_record(syn_arrays):
+    """Layout of a pathway's synapse records: the target index, the 4-byte variables,
+    a pad word if needed, the 8-byte variables.  Returns ([(field, ctype)], 32-bit
+    words, alignment); records of 8 or 16 bytes are aligned to their size, so that
+    one load reads a record."""
+    small = [(n, ct) for n, ct in syn_arrays.items() if ct in ("float", "int")]
+    big = [(n, ct) for n, ct in syn_arrays.items() if ct not in ("float", "int")]
+    fields = [("post", "int")] + small
+    words = len(fields)
+    if big:
+        if words % 2:
+            fields.append(("pad_", "int"))
+            words += 1
+        fields += big
+        words += 2 * len(big)
+    align = 4 * words if words in (2, 4) else (8 if big else 4)
+    return fields, words, align
+
+
 def rec_type(val_t):
     """Staged event record of a value type: 12 bytes for 4-byte values, 16 for 8-byte."""
     return "hb_rec8" if val_t in ("double", "long long") else "hb_rec4"
@@ -646,6 +665,11 @@ class GatherKernel:
         self._build()
 
     def _build(self):
+        self.rec_name = f"hb_syn_{self.name}"
+        fields, words, align = synapse_record(self.syn_arrays)
+        body = " ".join(f"{ct} {n};" for n, ct in fields)
+        self.rec_decl = (f"// one synapse: target and the on_pre statement's synaptic variables ({4 * words} bytes)\n"
+                         f"struct __attribute__((aligned({align}))) {self.rec_name} {{ {body} }};")
         ps = ParamStruct(f"hbg_{self.name}")
         ps.ptr("clock", "long long", const=True)
         ps.val("step_off", "long long")
@@ -656,9 +680,7 @@ class GatherKernel:
         ps.val("src0", "int")
         ps.val("src1", "int")
         ps.ptr("seg_ptr", "unsigned int", const=True)
-        ps.ptr("post", "int", const=True)
-        for v, ct in self.syn_arrays.items():
-            ps.ptr(f"s_{v}", ct, const=True)
+        ps.ptr("syn_rec", "unsigned int", const=True)
         for n, (_, ct) in sorted(self.scalars.items()):
             ps.val(f"c_{n}", ct)
         ps.ptr("stg_rec", "unsigned int")
@@ -689,7 +711,7 @@ class GatherKernel:
         c, _ = Renderer(types).render(self.on_pre[-1]["expr"])
         lines.append(f"hb_gv[hb_u] = ({VT})({c});")
         lines = _rename(lines, names)
-        k = [source_header(GT), ps.c_decl()]
+        k = [source_header(GT), ps.c_decl(), self.rec_decl]
         k.append(f'extern "C" __global__ void __launch_bounds__({GT}) {self.kernel_name}(const {ps.name} p) {{')
         k += [
             "    const long long step = p.clock[0] + p.step_off;",
@@ -787,9 +809,8 @@ class GatherKernel:
             "            // then all loads in flight at once, then keys and values",
             f"            unsigned int hb_ge[{U}];",
             f"            unsigned short hb_gs[{U}];",
-            f"            int hb_gp[{U}];",
+            f"            {self.rec_name} hb_gr[{U}];",
         ]
-        k += [f"            {ct} hb_sv_{v}[{U}];" for v, ct in self.syn_arrays.items()]
         k += [
             "            #pragma unroll",
             f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
@@ -802,20 +823,19 @@ class GatherKernel:
             "            }",
             "            #pragma unroll",
             f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
-            "                hb_gp[hb_u] = p.post[hb_ge[hb_u]];",
+            f"                hb_gr[hb_u] = ((const {self.rec_name}*)p.syn_rec)[hb_ge[hb_u]];",
         ]
-        k += [f"                hb_sv_{v}[hb_u] = p.s_{v}[hb_ge[hb_u]];" for v in self.syn_arrays]
         k += [
             "            }",
             "            #pragma unroll",
             f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
             "                if (hb_gt[hb_u] >= 0) {",
             "                    const unsigned int e = hb_ge[hb_u];",
-            "                    const int _postsynaptic_idx = hb_gp[hb_u];",
+            "                    const int _postsynaptic_idx = hb_gr[hb_u].post;",
             "                    const int _idx = (int)e;",
             "                    (void)_idx;",
         ]
-        k += [f"                    const {ct} {v} = hb_sv_{v}[hb_u];" for v, ct in self.syn_arrays.items()]
+        k += [f"                    const {ct} {v} = hb_gr[hb_u].{v};" for v, ct in self.syn_arrays.items()]
         k += ["                    " + x for x in lines]
         k += [
             f"                    const int tile = _postsynaptic_idx / {TT};",

@@ -25,9 +25,10 @@ from .kernels import (
     GatherKernel,
     NeuronKernel,
     fold_capacity,
-    rec_words,
     identifiers,
+    rec_words,
     scalar_values,
+    synapse_record,
 )
 
 L = _lib.lib
@@ -788,16 +789,15 @@ class Network:
                          pairs=s.pairs)
         s.kernel = K
         s.fn = D.function(K.source, K.kernel_name)
+        s.csr["rec"] = self._synapse_records(s, types_syn)
         cls = K.struct.ctypes_struct()
         s.struct_cls = cls
         v = {"clock": self.clock.data_ptr(), "step_off": 0, "ring_spk": s.source.ring_spk.data_ptr(),
              "ring_cnt": s.source.ring_cnt.data_ptr(), "ring_stride": s.source.N, "ring_slots": s.source.ring_slots,
              "src0": s.src_range[0], "src1": s.src_range[1], "seg_ptr": s.csr["seg_ptr"].data_ptr(),
-             "post": s.csr["post"].data_ptr(), "stg_rec": g.stg_rec.data_ptr(),
+             "syn_rec": s.csr["rec"].data_ptr(), "stg_rec": g.stg_rec.data_ptr(),
              "tile_off": g.tile_off.data_ptr(), "tile_cnt": g.tile_cnt.data_ptr(),
              "status": self.status.data_ptr(), "stg_stride": g.stg_cap}
-        for n in s.syn_vars:
-            v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
         for n, (val, ct) in sc.items():
             v[f"c_{n}"] = val
         nsrc = s.src_range[1] - s.src_range[0]
@@ -815,6 +815,26 @@ class Network:
         s.launch_index = len(self._launches)
         s.args, s.grid = a, grid
         self._add_launch(s.fn, grid, K.block, a, cls)
+
+    @staticmethod
+    def _synapse_records(s, types_syn):
+        """The gather's synapse records in CSR order (layout: kernels.synapse_record):
+        target index and the on_pre statement's synaptic variables side by side, so
+        that one load per synapse reads them from one cache line."""
+        torch = _torch()
+        fields, words, _ = synapse_record(types_syn)
+        S = int(s.csr["n_syn"])
+        if S == 0:
+            return torch.zeros(words, dtype=torch.int32, device="cuda")
+        cols = []
+        for name, ct in fields:
+            if name == "post":
+                cols.append(s.csr["post"][:S].view(torch.int32).view(S, 1))
+            elif name == "pad_":
+                cols.append(torch.zeros((S, 1), dtype=torch.int32, device="cuda"))
+            else:
+                cols.append(s.csr["syn"][name][:S].contiguous().view(torch.int32).view(S, -1))
+        return torch.cat(cols, dim=1).contiguous().view(-1)
 
     def _add_launch(self, fn, grid, block, args, cls, step_field="step_off"):
         off = getattr(cls, step_field).offset
