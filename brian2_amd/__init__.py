@@ -15,6 +15,8 @@ from .runtime import (  # noqa: F401
     Model,
     Network,
     NeuronGroup,
+    PoissonGroup,
+    PopulationRateMonitor,
     SpikeGeneratorGroup,
     SpikeMonitor,
     StateMonitor,

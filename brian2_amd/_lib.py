@@ -58,6 +58,7 @@ _SIGS = {
     "hb_plan_destroy": ([_vp], _i),
     "hb_plan_profile": ([_vp, _vp, _i64, _P(ctypes.c_float)], _i),
     "hb_spikegen_function": ([_P(_vp)], _i),
+    "hb_poisson_function": ([_P(_vp)], _i),
     "hb_csr_count": ([_vp, _vp, _i64, _i32, _i32, _i32, _i32, _vp, _vp], _i),
     "hb_scan_u32": ([_vp, _i64, _vp], _i),
     "hb_gen_fixed_indegree": ([_u64, _u64, _i32, _i32, _i32, _vp, _vp, _vp], _i),

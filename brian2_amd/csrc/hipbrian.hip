@@ -486,6 +486,83 @@ extern "C" int hb_spikegen_function(void** fn) {
 }
 
 // ---------------------------------------------------------------------------
+// PoissonGroup threshold (poissongroup.py:106, "rand() < rates * dt", run by
+// threshold.pyx:18-32) with a counter-based generator
+// ---------------------------------------------------------------------------
+struct hb_poisson_args {
+    const double* rdt;       // rates * dt per neuron (the product Brian's threshold evaluates)
+    const int64_t* clock;
+    int64_t step_offset;
+    uint64_t seed, stream;   // uniform of neuron i at step s: hb_rand64(seed, stream, s * N + i)
+    int32_t* ring_spk;       // [ring_slots][ring_stride]
+    int32_t* ring_cnt;       // [ring_slots]
+    int32_t ring_slots, ring_stride, N, pad;
+    // SpikeMonitor (mon_i == nullptr: none): (index, step) appended, per-neuron counts
+    int32_t* mon_i;
+    int32_t* mon_step;
+    uint64_t* mon_n;
+    int32_t* mon_count;
+    int64_t mon_cap;
+    int32_t mon_start, mon_stop;
+    // PopulationRateMonitor (rate_cnt == nullptr: none): spikes in [rate_start, rate_stop)
+    // of step s added to rate_cnt[s - rate_row0]
+    int32_t* rate_cnt;
+    int64_t rate_row0, rate_rows;
+    int32_t rate_start, rate_stop;
+    int32_t* status;         // bit 2: spike monitor full
+};
+
+__global__ void __launch_bounds__(256) hb_k_poisson(hb_poisson_args a) {
+    __shared__ int s_cnt[3];           // spikes, monitored spikes, rate-monitored spikes of the block
+    __shared__ int s_base;
+    __shared__ unsigned long long s_mbase;
+    const int64_t step = a.clock[0] + a.step_offset;
+    const int slot = (int)(step % a.ring_slots);
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+    const bool sp = i < a.N &&
+        hb_uniform01(hb_rand64(a.seed, a.stream, (uint64_t)step * (uint64_t)a.N + (uint64_t)i)) < a.rdt[i];
+    const bool mon = a.mon_i && sp && i >= a.mon_start && i < a.mon_stop;
+    const bool rate = a.rate_cnt && sp && i >= a.rate_start && i < a.rate_stop;
+    // offsets in the block: ballot prefix within the wave, one LDS atomic per wave
+    const unsigned long long b0 = __ballot(sp), b1 = __ballot(mon), b2 = __ballot(rate);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    __syncthreads();
+    int w0 = 0, w1 = 0;
+    if (lane == 0) {
+        if (b0) w0 = atomicAdd(&s_cnt[0], __popcll(b0));
+        if (b1) w1 = atomicAdd(&s_cnt[1], __popcll(b1));
+        if (b2) atomicAdd(&s_cnt[2], __popcll(b2));
+    }
+    w0 = __shfl(w0, 0, 64);
+    w1 = __shfl(w1, 0, 64);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_base = s_cnt[0] ? atomicAdd(&a.ring_cnt[slot], s_cnt[0]) : 0;
+        s_mbase = s_cnt[1] ? atomicAdd(a.mon_n, (unsigned long long)s_cnt[1]) : 0ull;
+        if (s_cnt[1] && s_mbase + (unsigned long long)s_cnt[1] > (unsigned long long)a.mon_cap) atomicOr(a.status, 2);
+        if (s_cnt[2] && step - a.rate_row0 >= 0 && step - a.rate_row0 < a.rate_rows)
+            atomicAdd(&a.rate_cnt[step - a.rate_row0], s_cnt[2]);
+        if (blockIdx.x == 0) a.ring_cnt[(step + 1) % a.ring_slots] = 0;
+    }
+    __syncthreads();
+    if (sp) a.ring_spk[(size_t)slot * a.ring_stride + s_base + w0 + __popcll(b0 & lt)] = i;
+    if (mon) {
+        const unsigned long long o = s_mbase + (unsigned long long)(w1 + __popcll(b1 & lt));
+        if (o < (unsigned long long)a.mon_cap) { a.mon_i[o] = i; a.mon_step[o] = (int32_t)step; }
+        a.mon_count[i - a.mon_start] += 1;
+    }
+}
+
+static hb_fn g_poisson_fn = {nullptr, (const void*)hb_k_poisson};
+
+extern "C" int hb_poisson_function(void** fn) {
+    *fn = &g_poisson_fn;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // prefix scan (uint32 counts -> n+1 offsets), three phases
 // ---------------------------------------------------------------------------
 #define HB_SCAN_BLOCK 1024  // elements per workgroup (256 threads x 4)

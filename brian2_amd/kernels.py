@@ -110,6 +110,7 @@ def _rename(lines, names):
 
 FOLD_EVENTS_PER_THREAD = 8    # fold: events a thread keeps in registers
 GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per window
+GATHER_STAGE_RUNS = True      # gather: sort a window's events by tile in LDS before writing them
 LDS_BYTES = 163840            # LDS of one CU
 
 
@@ -359,7 +360,7 @@ class NeuronKernel:
     def __init__(self, name, N, arrays, float_t, stateupdate=None, threshold=None, reset=None,
                  reset_mode="none", scalars=None, fold=None, lds_cap=0, val_t="double",
                  spikemon=None, statemon=None, flush=False, uses_refractory=False, statemon_when="start",
-                 tile=256, npt=1):
+                 tile=256, npt=1, ratemon=None):
         self.name = name
         self.N = N
         self.arrays = arrays          # {var: ctype} of the group's state arrays
@@ -373,6 +374,7 @@ class NeuronKernel:
         if self.cap < 128 or self.cap % 2 or fold_lds_bytes(tile, self.cap, val_t, npt) > LDS_BYTES:
             raise ValueError("fold capacity must be an even number of events, at least 128, whose LDS fits a CU")
         self.spikemon = spikemon      # (start, stop) or None
+        self.ratemon = None if flush else ratemon  # (start, stop) of a PopulationRateMonitor or None
         self.statemon = statemon      # [vars] or None
         self.statemon_when = statemon_when
         self.flush = flush
@@ -428,6 +430,12 @@ class NeuronKernel:
             ps.val("mon_cap", "long long")
             ps.val("mon_start", "int")
             ps.val("mon_stop", "int")
+        if self.ratemon:
+            ps.ptr("rate_cnt", "int")
+            ps.val("rate_row0", "long long")
+            ps.val("rate_rows", "long long")
+            ps.val("rate_start", "int")
+            ps.val("rate_stop", "int")
         if self.statemon:
             ps.ptr("rec_idx", "int", const=True)
             ps.ptr("rec_buf", "double")
@@ -456,7 +464,7 @@ class NeuronKernel:
         k.append("    (void)t; (void)dt;")
         if not self.flush:
             # spike counts of the workgroup (cleared here, before the first barrier)
-            k += ["    __shared__ int s_nsp, s_nspm;", "    if (threadIdx.x == 0) { s_nsp = 0; s_nspm = 0; }"]
+            k += ["    __shared__ int s_nsp, s_nspm, s_nrt;", "    if (threadIdx.x == 0) { s_nsp = 0; s_nspm = 0; s_nrt = 0; }"]
         fold = _fold_source(T, self.cap, self.val_t, self.flush, NPT) if self.fold else None
         if fold:
             k += fold[0]  # the tile's event count first: phase A's loads depend on it, not on the state
@@ -576,6 +584,17 @@ class NeuronKernel:
             k.append("        const int hb_off = hb_block_alloc(hb_n, &s_nsp);")
             if self.spikemon:
                 k.append("        const int hb_offm = hb_block_alloc(hb_nm, &s_nspm);")
+        if self.ratemon:
+            k += [
+                "        // spikes of the rate monitor's range: only their number is needed",
+                "        int hb_nr = 0;",
+                "        #pragma unroll",
+                f"        for (int q = 0; q < {NPT}; q++) {{",
+                f"            const int i = hb_i0 + q * {T};",
+                "            hb_nr += (hb_condq[q] && i >= p.rate_start && i < p.rate_stop) ? 1 : 0;",
+                "        }",
+                "        (void)hb_block_alloc(hb_nr, &s_nrt);",
+            ]
         k += [
             "        __syncthreads();",
             "        if (threadIdx.x == 0) {",
@@ -588,6 +607,11 @@ class NeuronKernel:
                 "            s_mbase = hb_nmtot ? atomicAdd(p.mon_n, (unsigned long long)hb_nmtot) : 0ull;",
                 "            if (hb_nmtot && s_mbase + (unsigned long long)hb_nmtot > (unsigned long long)p.mon_cap) "
                 "atomicOr(p.status, 2);",
+            ]
+        if self.ratemon:
+            k += [
+                "            if (s_nrt && p.rate_cnt && step - p.rate_row0 >= 0 && step - p.rate_row0 < p.rate_rows)",
+                "                atomicAdd(&p.rate_cnt[step - p.rate_row0], s_nrt);",
             ]
         k += [
             "        }",
@@ -729,7 +753,7 @@ class GatherKernel:
         lds = WN * 2 + NT * 8 + (NS + 1) * 4 + PC * 10 + 1024
         if lds > LDS_BYTES:
             raise NotImplementedError("too many target tiles for the gather's LDS")
-        self.staged = lds + WN * (8 + vb) <= LDS_BYTES
+        self.staged = GATHER_STAGE_RUNS and lds + WN * (8 + vb) <= LDS_BYTES
         self.lds_bytes = lds + (WN * (8 + vb) if self.staged else 0)
         # workgroups that share a CU: LDS, and 16 waves of at most 128 VGPRs
         self.wgs_per_cu = max(1, min(LDS_BYTES // self.lds_bytes, 1024 // GT))

@@ -32,6 +32,7 @@ from .kernels import (
 )
 
 L = _lib.lib
+S_POISSON_GROUP = 17  # generator stream of PoissonGroup draws (hb_rng.h)
 SPEC_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "specs")
 
 _NP = {"double": np.float64, "float": np.float32, "int": np.int32, "long long": np.int64, "bool": np.uint8}
@@ -255,6 +256,26 @@ class SpikeGeneratorGroup(_GroupBase):
         self.uses_refractory = False
 
 
+class PoissonGroup(_GroupBase):
+    """N independent Poisson spike sources (brian2/input/poissongroup.py:24): neuron i
+    spikes in a step when a uniform draw is below rates[i] * dt (poissongroup.py:106).
+    The draws come from a counter-based generator keyed by (seed, step, i), so the
+    spike trains depend on nothing but the seed and the step (numpy's generator, which
+    Brian uses, is not reproduced: agreement with Brian is statistical)."""
+
+    def __init__(self, N, rates, seed=1234, name=None):
+        self.N = int(N)
+        self.name = name or _unique("poissongroup")
+        self.rates = np.broadcast_to(np.asarray(rates, np.float64), (self.N,)).copy()
+        if np.any(self.rates < 0) or not np.all(np.isfinite(self.rates)):
+            raise ValueError("rates must be finite and non-negative")
+        self.seed = int(seed)
+        self.vars = {}
+        self._dev = None
+        self.net = None
+        self.uses_refractory = False
+
+
 class Synapses:
     """One synaptic object with an on_pre pathway.
 
@@ -467,6 +488,63 @@ class StateMonitor:
         return np.transpose(a[:, :, pos], (1, 2, 0))
 
 
+class PopulationRateMonitor:
+    """Population rate of a group or subgroup at every step, recorded after the
+    resets (brian2/monitors/ratemonitor.py:172, template ratemonitor.pyx:7-48):
+    rate = spikes / dt / neurons.  The device counts the spikes of each step; the
+    rates are formed on the host with the template's expression."""
+
+    def __init__(self, source, name=None):
+        if isinstance(source, Subgroup):
+            self.source, self.range = source.group, (source.start, source.stop)
+        else:
+            self.source, self.range = source, (0, source.N)
+        self.name = name or _unique("ratemonitor")
+        self._counts = []
+        self._steps = []
+        self.net = None
+
+    @property
+    def count(self):
+        """Spikes of the (sub)group per recorded step."""
+        return np.concatenate(self._counts) if self._counts else np.zeros(0, np.int64)
+
+    @property
+    def step(self):
+        return np.concatenate(self._steps) if self._steps else np.zeros(0, np.int64)
+
+    def t(self, dt):
+        return self.step * dt
+
+    def rate(self, dt):
+        """Hz, as ratemonitor.pyx:47: _num_spikes / _clock_dt / _num_source_neurons."""
+        n = self.range[1] - self.range[0]
+        return self.count.astype(np.float64) / dt / n
+
+    def smooth_rate(self, dt, window="gaussian", width=None):
+        """The rate convolved with a normalised window (ratemonitor.py:63-160):
+        "gaussian" (standard deviation `width`, cut at two of them), "flat" (`width`
+        rounded to an odd number of steps) or an array of odd length."""
+        if width is None and isinstance(window, str):
+            raise TypeError("Need a width when using a predefined window.")
+        if width is not None and not isinstance(window, str):
+            raise TypeError("Can only specify a width for a predefined window")
+        if isinstance(window, str):
+            if window == "gaussian":
+                width_dt = int(np.round(2 * width / dt))
+                window = np.exp(-(np.arange(-width_dt, width_dt + 1) ** 2) * 1.0 / (2 * (width / dt) ** 2))
+            elif window == "flat":
+                window = np.ones(int(np.round(width / (2 * dt))) * 2 + 1)
+            else:
+                raise NotImplementedError(f'Unknown pre-defined window "{window}"')
+        else:
+            window = np.asarray(window)
+            if window.ndim != 1 or len(window) % 2 != 1:
+                raise TypeError("The window has to be one-dimensional with an odd number of values.")
+        window = window * 1.0 / sum(window)
+        return np.convolve(self.rate(dt), window, mode="same")
+
+
 # ---------------------------------------------------------------------------
 class Network:
     """Runs the step loop on the device (Network.run, brian2/core/network.py:1047-1292)."""
@@ -510,10 +588,14 @@ class Network:
     def _build(self):
         torch = _torch()
         D = self.device = Device.get()
-        groups = [o for o in self.objs if isinstance(o, (NeuronGroup, SpikeGeneratorGroup))]
+        groups = [o for o in self.objs if isinstance(o, (NeuronGroup, SpikeGeneratorGroup, PoissonGroup))]
         paths = sorted([o for o in self.objs if isinstance(o, Synapses)], key=lambda s: (s.order, s.pathway_name))
         smons = [o for o in self.objs if isinstance(o, SpikeMonitor)]
         stmons = [o for o in self.objs if isinstance(o, StateMonitor)]
+        self.rmons = [o for o in self.objs if isinstance(o, PopulationRateMonitor)]
+        for m in smons + stmons + self.rmons:
+            if m.source not in groups:
+                raise RuntimeError(f"{m.source.name} recorded by {m.name} is not in the network")
         for s in paths:
             for g in (s.source, s.target):
                 if g not in groups:
@@ -543,7 +625,7 @@ class Network:
             if not incoming:
                 continue
             if not isinstance(g, NeuronGroup):
-                raise NotImplementedError("pathways into a spike generator")
+                raise NotImplementedError("pathways into a spike source")
             vts = {self._value_type(s) for s in incoming}
             if len(vts & {"int", "long long"}) and len(vts) > 1:
                 raise NotImplementedError("integer and floating-point pathway targets in one group")
@@ -555,8 +637,16 @@ class Network:
         _lib.check(L.hb_plan_create(D.ctx, ptr(self.clock), ctypes.byref(self._plan)), "hb_plan_create")
         self._launches = []
         for g in groups:
+            rm = [m for m in self.rmons if m.source is g]
+            if len(rm) > 1:
+                raise NotImplementedError("more than one PopulationRateMonitor per group")
+            g.rmon = rm[0] if rm else None
+            if g.rmon is not None:
+                g.rmon.buf = None
             if isinstance(g, NeuronGroup):
                 self._build_group(g)
+            elif isinstance(g, PoissonGroup):
+                self._build_poisson(g)
             else:
                 self._build_spikegen(g)
         for s in paths:
@@ -690,18 +780,13 @@ class Network:
                          spikemon=g.smon.range if g.smon else None,
                          statemon=g.stmon.variables if g.stmon else None,
                          uses_refractory=g.uses_refractory,
-                         statemon_when=g.stmon.when if g.stmon else "start", tile=g.block, npt=g.npt)
+                         statemon_when=g.stmon.when if g.stmon else "start", tile=g.block, npt=g.npt,
+                         ratemon=g.rmon.range if g.rmon else None)
         g.kernel = K
         g.fn = D.function(K.source, K.kernel_name)
         g.struct_cls = K.struct.ctypes_struct()
         if g.smon:
-            m = g.smon
-            m.cap = max(1 << 20, 2 * g.N * self.graph_steps)
-            m.mon_i = zeros(m.cap, "int")
-            m.mon_step = zeros(m.cap, "int")
-            m.mon_n = zeros(1, "long long")
-            m.count_dev = zeros(m.range[1] - m.range[0], "int")
-            m._drained = 0
+            self._alloc_spike_monitor(g, g.smon)
         if g.stmon:
             m = g.stmon
             m.rec_idx = dev(m.sorted_idx, "int")
@@ -721,7 +806,17 @@ class Network:
         else:
             g.flush_fn = None
 
+    def _alloc_spike_monitor(self, g, m):
+        m.cap = max(1 << 20, 2 * g.N * self.graph_steps)
+        m.mon_i = zeros(m.cap, "int")
+        m.mon_step = zeros(m.cap, "int")
+        m.mon_n = zeros(1, "long long")
+        m.count_dev = zeros(m.range[1] - m.range[0], "int")
+        m._drained = 0
+
     def _group_args(self, g, flush=False):
+        if isinstance(g, PoissonGroup):
+            return self._poisson_args(g)
         cls = g.flush_struct_cls if flush else g.struct_cls
         v = {"clock": self.clock.data_ptr(), "step_off": 0, "dt": self.dt, "N": g.N, "ring_slots": g.ring_slots,
              "ring_spk": g.ring_spk.data_ptr(), "ring_cnt": g.ring_cnt.data_ptr(), "ring_stride": g.N,
@@ -740,6 +835,10 @@ class Network:
             m = g.smon
             v.update(mon_i=m.mon_i.data_ptr(), mon_step=m.mon_step.data_ptr(), mon_n=m.mon_n.data_ptr(),
                      mon_count=m.count_dev.data_ptr(), mon_cap=m.cap, mon_start=m.range[0], mon_stop=m.range[1])
+        if not flush and g.rmon is not None:
+            m = g.rmon
+            v.update(rate_cnt=m.buf.data_ptr() if m.buf is not None else 0, rate_row0=getattr(m, "row0", 0),
+                     rate_rows=getattr(m, "rows", 0), rate_start=m.range[0], rate_stop=m.range[1])
         if g.stmon and (not flush or g.stmon.when == "end"):
             m = g.stmon
             v.update(rec_idx=m.rec_idx.data_ptr(), rec_buf=m.rec_buf.data_ptr() if m.rec_buf is not None else 0,
@@ -775,6 +874,48 @@ class Network:
         g.flush_fn = None
         g.smon = None
         g.stmon = None
+
+    class _PoissonArgs(ctypes.Structure):
+        """struct hb_poisson_args (hipbrian.hip)."""
+        _fields_ = [("rdt", ctypes.c_void_p), ("clock", ctypes.c_void_p), ("step_offset", ctypes.c_int64),
+                    ("seed", ctypes.c_uint64), ("stream", ctypes.c_uint64), ("ring_spk", ctypes.c_void_p),
+                    ("ring_cnt", ctypes.c_void_p), ("ring_slots", ctypes.c_int32), ("ring_stride", ctypes.c_int32),
+                    ("N", ctypes.c_int32), ("pad", ctypes.c_int32), ("mon_i", ctypes.c_void_p),
+                    ("mon_step", ctypes.c_void_p), ("mon_n", ctypes.c_void_p), ("mon_count", ctypes.c_void_p),
+                    ("mon_cap", ctypes.c_int64), ("mon_start", ctypes.c_int32), ("mon_stop", ctypes.c_int32),
+                    ("rate_cnt", ctypes.c_void_p), ("rate_row0", ctypes.c_int64), ("rate_rows", ctypes.c_int64),
+                    ("rate_start", ctypes.c_int32), ("rate_stop", ctypes.c_int32), ("status", ctypes.c_void_p)]
+
+    def _build_poisson(self, g):
+        fn = ctypes.c_void_p()
+        _lib.check(L.hb_poisson_function(ctypes.byref(fn)))
+        g.fn = fn
+        # poissongroup.py:106: the threshold compares rand() with rates * dt (an fp64 product)
+        g.rdt = dev(g.rates * self.dt, "double")
+        smon = [m for m in self.smons if m.source is g]
+        if len(smon) > 1:
+            raise NotImplementedError("more than one SpikeMonitor per group")
+        g.smon = smon[0] if smon else None
+        if g.smon:
+            self._alloc_spike_monitor(g, g.smon)
+        if any(m.source is g for m in self.stmons):
+            raise NotImplementedError("a PoissonGroup has no state variables to record")
+        g.stmon = None
+        g.struct_cls = self._PoissonArgs
+        g.launch_index = len(self._launches)
+        self._add_launch(fn, (g.N + 255) // 256, 256, self._poisson_args(g), g.struct_cls, step_field="step_offset")
+        g.flush_fn = None
+
+    def _poisson_args(self, g):
+        m, r = g.smon, g.rmon
+        return self._PoissonArgs(
+            g.rdt.data_ptr(), self.clock.data_ptr(), 0, g.seed, S_POISSON_GROUP, g.ring_spk.data_ptr(),
+            g.ring_cnt.data_ptr(), g.ring_slots, g.N, g.N, 0,
+            m.mon_i.data_ptr() if m else 0, m.mon_step.data_ptr() if m else 0, m.mon_n.data_ptr() if m else 0,
+            m.count_dev.data_ptr() if m else 0, m.cap if m else 0, m.range[0] if m else 0, m.range[1] if m else 0,
+            r.buf.data_ptr() if r is not None and r.buf is not None else 0, getattr(r, "row0", 0) if r else 0,
+            getattr(r, "rows", 0) if r else 0, r.range[0] if r else 0, r.range[1] if r else 0,
+            self.status.data_ptr())
 
     def _build_gather(self, s):
         D = self.device
@@ -883,6 +1024,9 @@ class Network:
             m = getattr(g, "stmon", None)
             if m is not None:
                 snap["monitors"][id(m)] = list(m._rows)
+            m = getattr(g, "rmon", None)
+            if m is not None:
+                snap["monitors"][id(m)] = (list(m._counts), list(m._steps))
         if not hasattr(self, "_stored"):
             self._stored = {}
         self._stored[name] = snap
@@ -903,6 +1047,10 @@ class Network:
             m = getattr(g, "stmon", None)
             if m is not None:
                 m._rows = list(snap["monitors"][id(m)])
+            m = getattr(g, "rmon", None)
+            if m is not None:
+                c, st = snap["monitors"][id(m)]
+                m._counts, m._steps = list(c), list(st)
         self.t_step = snap["t_step"]
 
     # -- run ------------------------------------------------------------------
@@ -924,6 +1072,12 @@ class Network:
                 m.row0 = self.t_step
                 m.rec_buf = torch.zeros(max(n_steps, 1) * len(m.variables) * len(m.sorted_idx), dtype=torch.float64,
                                         device="cuda")
+                self._set_launch_args(g.launch_index, self._group_args(g))
+            # population rate monitors: spike counts of this run's steps
+            m = getattr(g, "rmon", None)
+            if m is not None and not isinstance(g, SpikeGeneratorGroup):
+                m.rows, m.row0 = n_steps, self.t_step
+                m.buf = zeros(max(n_steps, 1), "int")
                 self._set_launch_args(g.launch_index, self._group_args(g))
         done = 0
         while done < n_steps:
@@ -955,6 +1109,15 @@ class Network:
             if m is not None and n_steps > 0:
                 nv, nr = len(m.variables), len(m.sorted_idx)
                 m._rows.append(m.rec_buf.view(n_steps, nv, nr).cpu().numpy().copy())
+            m = getattr(g, "rmon", None)
+            if m is not None and n_steps > 0:
+                first = self.t_step - n_steps
+                if isinstance(g, SpikeGeneratorGroup):
+                    counts = self._spikegen_counts(g, m.range, first, n_steps)
+                else:
+                    counts = m.buf[:n_steps].cpu().numpy().astype(np.int64)
+                m._counts.append(counts)
+                m._steps.append(np.arange(first, first + n_steps, dtype=np.int64))
         st = int(self.status.cpu().item())
         if st:
             raise _lib.HipBrianError(self._status_message(st))
@@ -974,6 +1137,8 @@ class Network:
                 self._ensure_monitor_room(g, m, int(n_steps))
             if getattr(g, "stmon", None) is not None:
                 raise NotImplementedError("profile() of a network with a StateMonitor")
+            if getattr(g, "rmon", None) is not None:
+                raise NotImplementedError("profile() of a network with a PopulationRateMonitor")
         ms = (ctypes.c_float * len(self._launches))()
         _lib.check(L.hb_plan_profile(self._plan, self.device.stream, int(n_steps), ms), "hb_plan_profile")
         self.t_step += int(n_steps)
@@ -984,6 +1149,23 @@ class Network:
         for obj in list(self.groups) + list(self.paths):
             kind = "gather" if isinstance(obj, Synapses) else "step"
             out.append((kind, obj.name, float(ms[obj.launch_index])))
+        return out
+
+    @staticmethod
+    def _spikegen_counts(g, rng, first, n):
+        """Spikes of a SpikeGeneratorGroup in [rng) per step first .. first + n - 1, from
+        its (bin, index) lists (the same bins the device replays)."""
+        sel = (g.indices >= rng[0]) & (g.indices < rng[1])
+        tb = g.timebins[sel].astype(np.int64)
+        steps = np.arange(first, first + n, dtype=np.int64)
+        if g.period_bins > 0:
+            steps = steps % g.period_bins
+        per_bin = np.bincount(tb, minlength=int(steps.max()) + 1 if n else 0) if len(tb) else None
+        if per_bin is None:
+            return np.zeros(n, np.int64)
+        out = np.zeros(n, np.int64)
+        ok = steps < len(per_bin)
+        out[ok] = per_bin[steps[ok]]
         return out
 
     def _status_message(self, st):

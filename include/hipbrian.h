@@ -17,6 +17,8 @@
  *  - hb_csr_* replace CSpikeQueue::prepare (synapses/spikequeue.h:48-105): per-source
  *    synapse lists in synapse order, with the integer delay (int)(d/dt + 0.5)
  *    (spikequeue.h:90) made a segment of each list.
+ *  - hb_poisson_function's kernel replaces PoissonGroup's Thresholder
+ *    (input/poissongroup.py:106 with codegen/runtime/cython_rt/templates/threshold.pyx).
  *  - hb_spikegen_function's kernel replaces the spikegenerator template
  *    (codegen/runtime/cython_rt/templates/spikegenerator.pyx:6-26).
  */
@@ -104,6 +106,17 @@ int hb_plan_destroy(hb_plan* plan);
  * slot (step mod ring_slots); `state` holds {lastindex}. period_bins > 0
  * repeats the pattern.  Launch from a plan with hb_spikegen_function. */
 int hb_spikegen_function(void** fn);
+/* PoissonGroup step (input/poissongroup.py:106: threshold "rand() < rates * dt",
+ * run by codegen/runtime/cython_rt/templates/threshold.pyx:18-32) for the step
+ * s = *clock + k: neuron i spikes when the uniform hb_rand64(seed, stream, s*N + i)
+ * mapped to [0, 1) is below rates[i] * dt (computed once on the host, the same fp64
+ * product), so a run is reproducible from (seed, step) alone and store/restore
+ * needs no generator state.  Spikes go to the ring slot (step mod ring_slots);
+ * optional SpikeMonitor (spikemonitor.pyx:9-58) and PopulationRateMonitor
+ * (ratemonitor.pyx:7-48) counts.  Launch from a plan with hb_poisson_function
+ * (argument block: struct hb_poisson_args in hipbrian.hip, 256 threads per
+ * workgroup). */
+int hb_poisson_function(void** fn);
 
 /* ---- connectivity / CSR (CSpikeQueue::prepare) -------------------------- */
 /* Counts synapses per (source, delay segment): seg_count[(pre-src0)*n_seg + d-dmin]

@@ -18,6 +18,9 @@
  *    statements Brian emitted (tests/golden/spec_*.json), and against the
  *    closed-form solution of the linear CUBA system.
  *  - generator: hand-checked values and statistical properties.
+ *  - PoissonGroup threshold (HBR_POISSON): the spike rate of every neuron within
+ *    binomial bounds of rates * dt, and independence across neurons and steps
+ *    (the reference draws with numpy, so only statistics can be compared).
  */
 #include "hbref.h"
 
@@ -498,7 +501,7 @@ int hbr_run(hbr_sim* S) {
         if (g->model == HBR_CUBA_EXACT) cuba_scalar(g->params, dt, &rt[gi].cuba);
         else if (g->model == HBR_COBAHH_EE) hh_scalar(g->params, dt, &rt[gi].hh);
         else if (g->model == HBR_LIF_EXACT) lif_scalar(g->params, dt, &rt[gi].lif);
-        else if (g->model != HBR_SPIKEGEN && g->model != HBR_PASSIVE) {
+        else if (g->model != HBR_SPIKEGEN && g->model != HBR_PASSIVE && g->model != HBR_POISSON) {
             free(rt);
             return fail("unknown model");
         }
@@ -543,6 +546,15 @@ int hbr_run(hbr_sim* S) {
             group_rt* R = &rt[gi];
             R->nspk = 0;
             if (g->model == HBR_PASSIVE) continue; /* "v : 1": no threshold */
+            if (g->model == HBR_POISSON) {
+                /* poissongroup.py:106: "rand() < rates * dt", the uniform drawn from the
+                   counter-based generator keyed by (seed, step, neuron) */
+                for (int32_t i = 0; i < g->N; i++)
+                    if (hbr_uniform(g->poisson_seed, g->poisson_stream, (uint64_t)step * (uint64_t)g->N + (uint64_t)i) <
+                        g->poisson_rdt[i])
+                        R->spikes[R->nspk++] = i;
+                continue;
+            }
             if (g->model == HBR_SPIKEGEN) {
                 int32_t timebin = (int32_t)step; /* t_in_timesteps */
                 int64_t idx;

@@ -42,7 +42,7 @@ int64_t hbr_queue_size(hbr_queue* q);
 int32_t hbr_queue_offset(hbr_queue* q);
 
 /* ---- network step restatement ---------------------------------------------- */
-enum { HBR_CUBA_EXACT = 1, HBR_COBAHH_EE = 2, HBR_LIF_EXACT = 3, HBR_SPIKEGEN = 4, HBR_PASSIVE = 5 };
+enum { HBR_CUBA_EXACT = 1, HBR_COBAHH_EE = 2, HBR_LIF_EXACT = 3, HBR_SPIKEGEN = 4, HBR_PASSIVE = 5, HBR_POISSON = 6 };
 
 typedef struct {
     int32_t model;          /* HBR_* */
@@ -56,6 +56,9 @@ typedef struct {
     const int32_t* gen_index;
     const int32_t* gen_timebin;
     int64_t gen_lastindex;  /* in/out */
+    /* HBR_POISSON: neuron i spikes at step s when hbr_uniform(seed, stream, s*N + i) < rdt[i] */
+    uint64_t poisson_seed, poisson_stream;
+    const double* poisson_rdt;
 } hbr_group;
 
 typedef struct {

@@ -15,14 +15,16 @@ LIB_PATH = os.path.join(ORACLE_DIR, "libhbref.so")
 i32, i64, u64, dbl, u8 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint8
 P = ctypes.POINTER
 
-HBR_CUBA_EXACT, HBR_COBAHH_EE, HBR_LIF_EXACT, HBR_SPIKEGEN, HBR_PASSIVE = 1, 2, 3, 4, 5
+HBR_CUBA_EXACT, HBR_COBAHH_EE, HBR_LIF_EXACT, HBR_SPIKEGEN, HBR_PASSIVE, HBR_POISSON = 1, 2, 3, 4, 5, 6
+S_POISSON_GROUP = 17  # generator stream of PoissonGroup draws
 
 
 class Group(ctypes.Structure):
     _fields_ = [
         ("model", i32), ("N", i32), ("state", P(dbl)), ("lastspike", P(dbl)),
         ("not_refractory", P(u8)), ("params", P(dbl)), ("n_gen", i64), ("gen_index", P(i32)),
-        ("gen_timebin", P(i32)), ("gen_lastindex", i64),
+        ("gen_timebin", P(i32)), ("gen_lastindex", i64), ("poisson_seed", u64), ("poisson_stream", u64),
+        ("poisson_rdt", P(dbl)),
     ]
 
 
@@ -171,7 +173,8 @@ class Queue:
 
 # ---------------------------------------------------------------------------
 class OracleGroup:
-    def __init__(self, model, N, state, params, lastspike=None, not_refractory=None, gen=None, refractory=True):
+    def __init__(self, model, N, state, params, lastspike=None, not_refractory=None, gen=None, refractory=True,
+                 poisson=None):
         self.model = model
         self.N = N
         self.state = np.ascontiguousarray(state, np.float64)
@@ -183,6 +186,7 @@ class OracleGroup:
             self.lastspike = self.not_refractory = None
         self.gen = gen  # (index, timebin) for spike generators
         self.gen_lastindex = 0
+        self.poisson = poisson  # (seed, rates * dt) for Poisson groups
 
     def struct(self):
         g = Group()
@@ -198,6 +202,10 @@ class OracleGroup:
             g.gen_index = ptr(self._gi, i32)
             g.gen_timebin = ptr(self._gt, i32)
         g.gen_lastindex = self.gen_lastindex
+        if self.poisson is not None:
+            self._rdt = np.ascontiguousarray(self.poisson[1], np.float64)
+            g.poisson_seed, g.poisson_stream = self.poisson[0], S_POISSON_GROUP
+            g.poisson_rdt = ptr(self._rdt, dbl)
         return g
 
 

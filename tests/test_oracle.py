@@ -396,3 +396,31 @@ def test_oracle_lif_equals_brian_golden():
         assert np.array_equal(si, z["i"][sel]) and np.array_equal(ss, z["step"][sel])
         assert np.array_equal(G.state, z["final"][b])
         assert np.array_equal(np.transpose(rec, (1, 2, 0)), z["trace"][b])
+
+
+def test_poisson_oracle_rates_and_independence():
+    """The oracle's PoissonGroup threshold (rand() < rates * dt, poissongroup.py:106) against
+    what the mathematics fixes: per-neuron counts within binomial bounds of n * rates * dt,
+    z-scores of mean 0 and spread 1 over the population, and no excess of spikes in
+    consecutive steps (independent draws); a rates/dt mix-up or a reversed comparison
+    fails these bounds, and a run started later must give the same trains (the draw
+    of a neuron depends on the seed and the absolute step only)."""
+    N, steps, seed, dt = 2000, 20000, 31, 1e-4
+    rates = 5.0 + 195.0 * O.uniform(5, 3, N)
+    g = O.OracleGroup(O.HBR_POISSON, N, np.zeros(N), [], refractory=False, poisson=(seed, rates * dt))
+    i, s = O.run([g], [], dt, steps, mon_group=0)[:2]
+    p = rates * dt
+    cnt = np.bincount(i, minlength=N)
+    z = (cnt - steps * p) / np.sqrt(steps * p * (1 - p))
+    assert np.all(np.abs(z) < 5.5)
+    assert abs(z.mean()) < 0.15 and 0.9 < z.std() < 1.1
+    order = np.lexsort((s, i))
+    i2, s2 = i[order], s[order]
+    consecutive = np.count_nonzero((i2[1:] == i2[:-1]) & (s2[1:] == s2[:-1] + 1))
+    expect = np.sum((steps - 1) * p * p)
+    assert abs(consecutive - expect) < 5 * np.sqrt(expect)
+    # the same trains from a later start step: draws depend on (seed, step, neuron) only
+    g2 = O.OracleGroup(O.HBR_POISSON, N, np.zeros(N), [], refractory=False, poisson=(seed, rates * dt))
+    i3, s3 = O.run([g2], [], dt, 500, start_step=7000, mon_group=0)[:2]
+    sel = (s >= 7000) & (s < 7500)
+    assert np.array_equal(i3, i[sel]) and np.array_equal(s3, s[sel])
