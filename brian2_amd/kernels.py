@@ -29,7 +29,15 @@ No floating-point atomics are used, so the sums are bit-identical to Brian's.
 """
 import re
 
-from .codegen import ParamStruct, Renderer, c_literal, ctype_of, source_header, translate_block
+from .codegen import (
+    ParamStruct,
+    Renderer,
+    c_literal,
+    ctype_of,
+    source_header,
+    translate_block,
+    translate_statement,
+)
 
 KEY_TGT_SHIFT = 48  # event key: [63:48] target in tile, [47:44] rank, [43:32] delay, [31:0] CSR position
 MAX_RANK = 16
@@ -659,7 +667,7 @@ class GatherKernel:
     consecutive positions)."""
 
     def __init__(self, name, rank, n_seg, dmin, on_pre, syn_arrays, scalars, val_t, float_t, target_var,
-                 n_tiles, tile, count_synapses=False, pairs=256):
+                 n_tiles, tile, count_synapses=False, pairs=256, target_index=None, plastic=()):
         if rank >= MAX_RANK:
             raise NotImplementedError(f"more than {MAX_RANK} pathways into one group")
         if n_seg > MAX_SEG:
@@ -675,7 +683,9 @@ class GatherKernel:
         self.rank = rank
         self.n_seg = n_seg
         self.dmin = dmin
-        self.on_pre = on_pre          # list of statement dicts, last one writes the target
+        self.on_pre = on_pre          # list of statement dicts, one of them writes the target
+        self.target_index = len(on_pre) - 1 if target_index is None else int(target_index)
+        self.plastic = set(plastic)   # synaptic variables the statements write (read and written in place)
         self.syn_arrays = syn_arrays  # {name: ctype}
         self.scalars = scalars        # {name: (value, ctype)}
         self.val_t = val_t
@@ -690,7 +700,10 @@ class GatherKernel:
 
     def _build(self):
         self.rec_name = f"hb_syn_{self.name}"
-        fields, words, align = synapse_record(self.syn_arrays)
+        # synaptic variables that the statements write are not copied into the records
+        self.rec_arrays = {} if self.plastic else dict(self.syn_arrays)  # the record layout's variables
+        fields, words, align = synapse_record(self.rec_arrays)
+        self.rec_words = words
         body = " ".join(f"{ct} {n};" for n, ct in fields)
         self.rec_decl = (f"// one synapse: target and the on_pre statement's synaptic variables ({4 * words} bytes)\n"
                          f"struct __attribute__((aligned({align}))) {self.rec_name} {{ {body} }};")
@@ -705,6 +718,10 @@ class GatherKernel:
         ps.val("src1", "int")
         ps.ptr("seg_ptr", "unsigned int", const=True)
         ps.ptr("syn_rec", "unsigned int", const=True)
+        if self.plastic:
+            ps.val("dt", "double")
+            for v, ct in self.syn_arrays.items():
+                ps.ptr(f"s_{v}", ct)
         for n, (_, ct) in sorted(self.scalars.items()):
             ps.val(f"c_{n}", ct)
         ps.ptr("stg_rec", "unsigned int")
@@ -724,16 +741,23 @@ class GatherKernel:
         NT = self.n_tiles
         TT = self.tile
         VT = self.val_t
-        # value of the on_pre statement (temporaries first)
+        # the on_pre statements in order; the target update's value goes to hb_gv (the fold
+        # applies it), the other statements run as they are
         types = dict(self.syn_arrays)
         types["_postsynaptic_idx"] = "int"
         types["_idx"] = "int"
+        types["t"] = "double"
         for n, (_, ct) in self.scalars.items():
             types[n] = ct
         names = {n: f"p.c_{n}" for n in self.scalars}
-        lines = translate_block(self.on_pre[:-1], {}, types, set(), self.float_t)
-        c, _ = Renderer(types).render(self.on_pre[-1]["expr"])
-        lines.append(f"hb_gv[hb_u] = ({VT})({c});")
+        lines = []
+        declared = set()
+        for i_, st in enumerate(self.on_pre):
+            if i_ == self.target_index:
+                c, _ = Renderer(types).render(st["expr"])
+                lines.append(f"hb_gv[hb_u] = ({VT})({c});")
+            else:
+                lines += translate_statement(st, types, declared, self.float_t)
         lines = _rename(lines, names)
         k = [source_header(GT), ps.c_decl(), self.rec_decl]
         k.append(f'extern "C" __global__ void __launch_bounds__({GT}) {self.kernel_name}(const {ps.name} p) {{')
@@ -817,7 +841,6 @@ class GatherKernel:
             f"        if ((int)threadIdx.x < {PC}) {{ s_a[threadIdx.x] = a; s_o[threadIdx.x] = ex; s_sg[threadIdx.x] = (unsigned short)seg; }}",
             "        __syncthreads();",
             f"        for (int w0 = 0; w0 < E; w0 += {WN}) {{",
-            f"            const int Ew = E - w0 < {WN} ? E - w0 : {WN};",
             "            // each pair writes its index over its events of the window",
             "            if ((int)threadIdx.x < nq) {",
             "                const int a_ = ex > w0 ? ex : w0;",
@@ -859,11 +882,18 @@ class GatherKernel:
             "                    const int _idx = (int)e;",
             "                    (void)_idx;",
         ]
-        k += [f"                    const {ct} {v} = hb_gr[hb_u].{v};" for v, ct in self.syn_arrays.items()]
+        if self.plastic:
+            k.append("                    const double t = (double)step * p.dt;")
+            k += [f"                    {ct} {v} = p.s_{v}[e];" for v, ct in self.syn_arrays.items()]
+        else:
+            k += [f"                    const {ct} {v} = hb_gr[hb_u].{v};" for v, ct in self.syn_arrays.items()]
         k += ["                    " + x for x in lines]
+        k += [f"                    p.s_{v}[e] = {v};" for v in self.syn_arrays if v in self.plastic]
         k += [
             f"                    const int tile = _postsynaptic_idx / {TT};",
             "                    hb_gt[hb_u] = tile;",
+            "                    // a target outside the tiles (corrupt synapse data) is reported, not staged",
+            f"                    if ((unsigned int)_postsynaptic_idx >= {NT * TT}u) {{ hb_gt[hb_u] = -1; atomicOr(p.status, 8); }}",
             f"                    hb_gk[hb_u] = ((unsigned long long)(_postsynaptic_idx - tile * {TT}) << {KEY_TGT_SHIFT})",
             f"                        | ((unsigned long long){self.rank} << 44) | ((unsigned long long)({NS - 1} - hb_gs[hb_u]) << 32)",
             "                        | (unsigned long long)e;",
@@ -918,8 +948,9 @@ class GatherKernel:
                 "            }",
                 "            __syncthreads();",
                 f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
-                "            // s_hist[t] is now the end of tile t's run in the window, s_hist[t - 1] its start",
-                f"            for (int x = threadIdx.x; x < Ew; x += {GT}) {{",
+                "            // s_hist[t] is now the end of tile t's run in the window, s_hist[t - 1] its start;",
+                "            // carry = the events placed (the window's events with a valid target)",
+                f"            for (int x = threadIdx.x; x < carry; x += {GT}) {{",
                 "                const int t = s_pid[x];",
                 "                const unsigned long long r = (unsigned long long)s_base[t] + (unsigned int)(x - (t > 0 ? s_hist[t - 1] : 0));",
                 "                if (HB_FITS(t, r)) {",
@@ -966,4 +997,70 @@ class GatherKernel:
         self.source = "\n".join(k) + "\n"
 
 
-__all__ = ["CodeSpec", "NeuronKernel", "GatherKernel", "scalar_values", "identifiers", "c_literal"]
+class PostPathKernel:
+    """A pathway's on_post code (brian2/synapses/synapses.py:205-216; template
+    synapses.pyx:16-53 run on the post pathway's queue): for every spike of the
+    target group at step s - delay, the statements run for each of the neuron's
+    incoming synapses (CSR by target, synapse order).  The statements may read
+    and write synaptic variables only, so synapses are independent: one thread
+    per synapse, one workgroup per spike at a time."""
+
+    BLOCK = 256
+
+    def __init__(self, name, statements, syn_arrays, plastic, scalars, float_t, delay_steps):
+        self.name = name
+        self.statements = statements
+        self.syn_arrays = syn_arrays
+        self.plastic = set(plastic)
+        self.scalars = scalars
+        self.float_t = float_t
+        self.delay = int(delay_steps)
+        self.kernel_name = f"post_{name}"
+        ps = ParamStruct(f"hbo_{name}")
+        ps.ptr("clock", "long long", const=True)
+        ps.val("step_off", "long long")
+        ps.val("dt", "double")
+        ps.ptr("ring_spk", "int", const=True)
+        ps.ptr("ring_cnt", "int", const=True)
+        ps.val("ring_stride", "long long")
+        ps.val("ring_slots", "int")
+        ps.val("pad", "int")
+        ps.ptr("post_ptr", "unsigned int", const=True)
+        ps.ptr("post_syn", "unsigned int", const=True)
+        for v, ct in syn_arrays.items():
+            ps.ptr(f"s_{v}", ct)
+        for n, (_, ct) in sorted(scalars.items()):
+            ps.val(f"c_{n}", ct)
+        self.struct = ps
+        types = dict(syn_arrays)
+        types["_idx"] = "int"
+        types["t"] = "double"
+        for n, (_, ct) in scalars.items():
+            types[n] = ct
+        lines = _rename(translate_block(statements, {}, types, set(), float_t), {n: f"p.c_{n}" for n in scalars})
+        B = self.BLOCK
+        k = [source_header(B), ps.c_decl()]
+        k.append(f'extern "C" __global__ void __launch_bounds__({B}) {self.kernel_name}(const {ps.name} p) {{')
+        k += [
+            "    const long long step = p.clock[0] + p.step_off;",
+            "    const double t = (double)step * p.dt;",
+            f"    const long long ps_ = step - {self.delay};",
+            "    if (ps_ < 0) return;",
+            "    const int slot = (int)(ps_ % p.ring_slots);",
+            "    const int P = p.ring_cnt[slot];",
+            "    for (int c = blockIdx.x; c < P; c += gridDim.x) {",
+            "        const int j = p.ring_spk[(size_t)slot * p.ring_stride + c];",
+            "        const unsigned int lo = p.post_ptr[j], hi = p.post_ptr[j + 1];",
+            f"        for (unsigned int k_ = lo + threadIdx.x; k_ < hi; k_ += {B}) {{",
+            "            const unsigned int e = p.post_syn[k_];",
+            "            const int _idx = (int)e;",
+            "            (void)_idx; (void)t;",
+        ]
+        k += [f"            {ct} {v} = p.s_{v}[e];" for v, ct in syn_arrays.items()]
+        k += ["            " + x for x in lines]
+        k += [f"            p.s_{v}[e] = {v};" for v in syn_arrays if v in self.plastic]
+        k += ["        }", "    }", "}"]
+        self.source = "\n".join(k) + "\n"
+
+
+__all__ = ["CodeSpec", "NeuronKernel", "GatherKernel", "PostPathKernel", "scalar_values", "identifiers", "c_literal"]

@@ -24,6 +24,7 @@ from .kernels import (
     CodeSpec,
     GatherKernel,
     NeuronKernel,
+    PostPathKernel,
     fold_capacity,
     identifiers,
     rec_words,
@@ -277,20 +278,29 @@ class PoissonGroup(_GroupBase):
 
 
 class Synapses:
-    """One synaptic object with an on_pre pathway.
+    """One synaptic object with an on_pre pathway and optionally an on_post one.
 
-    ``model`` is the pathway's synapses code object (CodeSpec).  Connectivity is
-    given explicitly with connect(i, j) (brian2/synapses/synapses.py:1704-1710,
-    synapse index = array order), per-synapse variables with set(), and delays
-    in seconds with delay= (scalar or per synapse).  ``storage`` may store a
-    float64 synaptic variable as float32 on the device when every value is
-    exactly representable (checked)."""
+    ``model`` is the on_pre pathway's synapses code object (CodeSpec): one
+    update of a postsynaptic variable, any number of updates of synaptic
+    variables (plasticity, event-driven traces: brian2/synapses/synapses.py:
+    359-383), temporaries.  ``on_post`` is the on_post pathway's code object
+    (synaptic variables only), run for the synapses of every postsynaptic spike
+    after the on_pre pathway of the same step (synapses.py:205-216: order -1
+    before 1), with a scalar delay in seconds ``post_delay``.  Connectivity is
+    given explicitly with connect(i, j) (synapses.py:1704-1710, synapse index =
+    array order), per-synapse variables with set() (unset ones start at 0, as in
+    Brian), and delays in seconds with set_delay (scalar or per synapse).
+    ``storage`` may store a float64 synaptic variable as float32 on the device
+    when every value is exactly representable (checked)."""
 
-    def __init__(self, source, target, model, name=None, storage=None, order=0, count_events=False):
+    def __init__(self, source, target, model, name=None, storage=None, order=0, count_events=False, on_post=None,
+                 post_delay=0.0):
         self.name = name or _unique("synapses")
         self.count_events = count_events
         self.pathway_name = f"{self.name}_pre"
         self.order = order
+        self.post_spec = on_post
+        self.post_delay = float(post_delay)
         if isinstance(source, Subgroup):
             self.source, self.src_range = source.group, (source.start, source.stop)
         else:
@@ -308,31 +318,54 @@ class Synapses:
         self.csr = None
         self.net = None
 
+    @staticmethod
+    def _synaptic_vars(spec, skip, what):
+        """Synaptic variables the statements use (reads of neuron variables are not
+        supported: propagation runs beside the group updates)."""
+        out = set()
+        for n in identifiers(spec.vector) - set(skip):
+            v = spec.variables.get(n)
+            if v is None or v["kind"] in ("Constant", "Function", "AuxiliaryVariable") or n in ("t", "dt"):
+                continue
+            ix = spec.indices.get(n, "_idx")
+            if ix != "_idx":
+                raise NotImplementedError(f"{what} code reads {n!r} of a neuron group")
+            if n == "_postsynaptic_idx":
+                continue
+            out.add(n)
+        return out
+
+    @staticmethod
+    def _synaptic_writes(spec, skip_index, what):
+        out = set()
+        for k, st in enumerate(spec.vector):
+            if k == skip_index or st["op"] == ":=":
+                continue
+            if spec.indices.get(st["var"], "_idx") != "_idx":
+                raise NotImplementedError(f"{what} code may update one postsynaptic variable only")
+            out.add(st["var"])
+        return out
+
     def _analyse(self):
         vs = self.spec.vector
         if not vs:
             raise NotImplementedError("empty on_pre code")
-        last = vs[-1]
         idx = self.spec.indices
-        if idx.get(last["var"]) != "_postsynaptic_idx" or last["op"] not in ("+=", "-=", "*=", "/=", "="):
-            raise NotImplementedError("on_pre code must end with an update of a postsynaptic variable")
-        for st in vs[:-1]:
-            if st["op"] != ":=":
-                raise NotImplementedError("on_pre code may only define temporaries before the target update")
-        used = identifiers(vs) - {last["var"]}
-        self.syn_vars = []
-        for n in sorted(used):
-            v = self.spec.variables.get(n)
-            if v is None or v["kind"] in ("Constant", "Function", "AuxiliaryVariable"):
-                continue
-            ix = idx.get(n, "_idx")
-            if ix == "_postsynaptic_idx" or ix == "_presynaptic_idx":
-                raise NotImplementedError(f"on_pre expression reads {n!r} of a neuron group")
-            if n == "_postsynaptic_idx":
-                continue
-            self.syn_vars.append(n)
-        self.target_var = last["var"]
-        self.op = last["op"]
+        tgt = [k for k, st in enumerate(vs) if idx.get(st["var"]) == "_postsynaptic_idx"]
+        if len(tgt) != 1 or vs[tgt[0]]["op"] not in ("+=", "-=", "*=", "/=", "="):
+            raise NotImplementedError("on_pre code must update exactly one postsynaptic variable")
+        self.target_index = tgt[0]
+        self.target_var = vs[tgt[0]]["var"]
+        self.op = vs[tgt[0]]["op"]
+        # synaptic variables written by either pathway (plasticity) are read and written in place
+        self.plastic = self._synaptic_writes(self.spec, self.target_index, "on_pre")
+        used = self._synaptic_vars(self.spec, {self.target_var}, "on_pre")
+        if self.post_spec is not None:
+            if any(self.post_spec.indices.get(st["var"], "_idx") != "_idx" for st in self.post_spec.vector):
+                raise NotImplementedError("on_post code may update synaptic variables only")
+            self.plastic |= self._synaptic_writes(self.post_spec, -1, "on_post")
+            used |= self._synaptic_vars(self.post_spec, (), "on_post")
+        self.syn_vars = sorted(used | self.plastic)
 
     def connect(self, i, j):
         i = np.asarray(i, np.int64)
@@ -353,8 +386,26 @@ class Synapses:
         self.delay = delay
 
     def connect_device_csr(self, seg_ptr, post, syn, dmin, n_seg, n_syn):
-        """Connectivity built on the device (see synth.py): CSR by (source, delay)."""
+        """Connectivity built on the device (see synth.py): CSR by (source, delay);
+        synapse index = storage position."""
+        if self.post_spec is not None:
+            raise NotImplementedError("on_post pathways need connect(i, j)")
         self.csr = dict(seg_ptr=seg_ptr, post=post, syn=syn, dmin=int(dmin), n_seg=int(n_seg), n_syn=int(n_syn))
+
+    def get(self, name):
+        """Values of a synaptic variable in synapse order (after a run: as updated by
+        the synapses' code)."""
+        if self.csr is None:
+            if name in self.syn:
+                return self.syn[name].copy()
+            raise RuntimeError("not built yet")
+        t = self.csr["syn"][name][: self.csr["n_syn"]].cpu().numpy().astype(np.float64)
+        order = self.csr.get("order")
+        if order is None:
+            return t
+        out = np.empty_like(t)
+        out[order] = t
+        return out
 
     # CSpikeQueue::prepare counterpart (spikequeue.h:48-105) on the host
     def _build_csr(self, dt):
@@ -380,10 +431,10 @@ class Synapses:
             raise NotImplementedError("more than 2^32 - 1 synapses in one pathway")
         syn = {}
         for n in self.syn_vars:
-            if n not in self.syn:
-                raise RuntimeError(f"synaptic variable {n!r} of {self.name} was not set")
-            v = self.syn[n][order]
+            v = self.syn[n][order] if n in self.syn else np.zeros(S)  # Brian's variables start at 0
             st = self.storage.get(n)
+            if st is not None and np.dtype(st) == np.float32 and n in self.plastic:
+                raise ValueError(f"{n!r} is written by the synapses' code: it cannot be stored as float32")
             if st is not None and np.dtype(st) == np.float32:
                 v32 = v.astype(np.float32)
                 if not np.array_equal(v32.astype(np.float64), v):
@@ -393,6 +444,15 @@ class Synapses:
                 syn[n] = dev(v)
         self.csr = dict(seg_ptr=dev(seg_ptr.astype(np.uint32).view(np.int32)), post=dev(self.post[order]),
                         syn=syn, dmin=dmin, n_seg=n_seg, n_syn=S, order=order)
+        if self.post_spec is not None:
+            # CSR by target for the on_post pathway: storage positions of each target's
+            # synapses in synapse order (the post pathway's queue order)
+            pos_post = self.post[order].astype(np.int64)
+            perm = np.lexsort((order, pos_post))
+            ptr_ = np.zeros(self.target.N + 1, np.uint64)
+            np.cumsum(np.bincount(pos_post, minlength=self.target.N), out=ptr_[1:])
+            self.csr["post_ptr"] = dev(ptr_.astype(np.uint32).view(np.int32))
+            self.csr["post_syn"] = dev(perm.astype(np.int32))
 
     @property
     def n_events(self):
@@ -606,13 +666,15 @@ class Network:
         self.status = torch.zeros(1, dtype=torch.int32, device="cuda")
         for s in paths:
             s._build_csr(self.dt)
+            s.post_delay_steps = int(s.post_delay / self.dt + 0.5)  # spikequeue.h:90
         # per group: device state, spike ring
         for g in groups:
             # threads per workgroup, neurons per thread; a workgroup owns a tile of block * npt neurons
             g.block = int(self.tile or (1024 if g.N >= 65536 else 256))
             g.npt = int(self.npt or 1)
             g.tile = g.block * g.npt
-            dmax = max([s.csr["dmin"] + s.csr["n_seg"] - 1 for s in paths if s.source is g] + [0])
+            dmax = max([s.csr["dmin"] + s.csr["n_seg"] - 1 for s in paths if s.source is g]
+                       + [s.post_delay_steps for s in paths if s.target is g and s.post_spec is not None] + [0])
             g.ring_slots = dmax + 2
             g.ring_spk = zeros(g.ring_slots * g.N, "int")
             g.ring_cnt = zeros(g.ring_slots, "int")
@@ -651,10 +713,15 @@ class Network:
                 self._build_spikegen(g)
         for s in paths:
             self._build_gather(s)
+        # on_post pathways after every on_pre pathway of the step (synapses.py:205-216)
+        for s in paths:
+            if s.post_spec is not None:
+                self._build_post(s)
         # with every delay at least one step, the gathers of step s read only spikes of earlier
         # steps: they run concurrently with the group updates of step s and with each other
         # (three side lanes, taken in turn)
-        self.concurrent_gathers = bool(paths) and all(s.csr["dmin"] >= 1 for s in paths) and self.overlap
+        self.concurrent_gathers = (bool(paths) and all(s.csr["dmin"] >= 1 for s in paths) and self.overlap
+                                   and not any(s.plastic for s in paths))
         if self.concurrent_gathers:
             for i, s in enumerate(paths):
                 _lib.check(L.hb_plan_set_lane(self._plan, s.launch_index, 1 + i % 3), "hb_plan_set_lane")
@@ -717,14 +784,16 @@ class Network:
         from .codegen import Renderer
 
         types = dict(s.syn_ctypes())
+        types["t"] = "double"
         for n, v in s.spec.variables.items():
             if v["kind"] == "Constant":
                 types[n] = "double"
         for st in s.spec.scalar:
             types[st["var"]] = ctype_of(st["dtype"], s.target.float_t, st["var"])
-        for st in s.spec.vector[:-1]:
-            types[st["var"]] = ctype_of(st["dtype"], s.target.float_t, st["var"])
-        _, t = Renderer(types).render(s.spec.vector[-1]["expr"])
+        for k, st in enumerate(s.spec.vector):
+            if k != s.target_index:
+                types.setdefault(st["var"], ctype_of(st["dtype"], s.target.float_t, st["var"]))
+        _, t = Renderer(types).render(s.spec.vector[s.target_index]["expr"])
         tv = s.target.vars[s.target_var]
         if tv in ("int", "long long", "bool"):
             return "long long" if tv == "long long" else "int"
@@ -927,10 +996,12 @@ class Network:
         types_syn = s.syn_ctypes()
         K = GatherKernel(s.name, rank, s.csr["n_seg"], s.csr["dmin"], s.spec.vector, types_syn, sc, g.val_t,
                          g.float_t, s.target_var, g.n_tiles, g.tile, count_synapses=s.count_events,
-                         pairs=s.pairs)
+                         pairs=s.pairs, target_index=s.target_index, plastic=s.plastic)
         s.kernel = K
         s.fn = D.function(K.source, K.kernel_name)
-        s.csr["rec"] = self._synapse_records(s, types_syn)
+        s.csr["rec"] = self._synapse_records(s, K.rec_arrays)
+        if s.csr["rec"].numel() != max(1, s.csr["n_syn"]) * K.rec_words and s.csr["n_syn"] > 0:
+            raise RuntimeError("synapse records do not match the gather's layout")
         cls = K.struct.ctypes_struct()
         s.struct_cls = cls
         v = {"clock": self.clock.data_ptr(), "step_off": 0, "ring_spk": s.source.ring_spk.data_ptr(),
@@ -941,7 +1012,10 @@ class Network:
              "status": self.status.data_ptr(), "stg_stride": g.stg_cap}
         for n, (val, ct) in sc.items():
             v[f"c_{n}"] = val
-        nsrc = s.src_range[1] - s.src_range[0]
+        if s.plastic:
+            v["dt"] = self.dt
+            for n in s.syn_vars:
+                v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
         # enough workgroups to fill every CU (units are taken in turn)
         grid = int(min(self.device.n_cu * K.wgs_per_cu, max(1, s.units_max)))
         s.events_dev = _torch().zeros(grid, dtype=_torch().int64, device="cuda")
@@ -956,6 +1030,30 @@ class Network:
         s.launch_index = len(self._launches)
         s.args, s.grid = a, grid
         self._add_launch(s.fn, grid, K.block, a, cls)
+
+    def _build_post(self, s):
+        host = {"dt": self.dt}
+        host.update(getattr(s, "namespace", {}) or {})
+        sc = scalar_values(s.post_spec, host, s.target.float_t)
+        K = PostPathKernel(s.name, s.post_spec.vector, s.syn_ctypes(), s.plastic, sc, s.target.float_t,
+                           s.post_delay_steps)
+        s.post_kernel = K
+        s.post_fn = self.device.function(K.source, K.kernel_name)
+        cls = K.struct.ctypes_struct()
+        g = s.target
+        v = {"clock": self.clock.data_ptr(), "step_off": 0, "dt": self.dt, "ring_spk": g.ring_spk.data_ptr(),
+             "ring_cnt": g.ring_cnt.data_ptr(), "ring_stride": g.N, "ring_slots": g.ring_slots,
+             "post_ptr": s.csr["post_ptr"].data_ptr(), "post_syn": s.csr["post_syn"].data_ptr()}
+        for n in s.syn_vars:
+            v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
+        for n, (val, ct) in sc.items():
+            v[f"c_{n}"] = val
+        a = cls()
+        for name, _ in cls._fields_:
+            if name in v:
+                setattr(a, name, v[name])
+        s.post_launch_index = len(self._launches)
+        self._add_launch(s.post_fn, int(min(256, max(1, g.N))), K.BLOCK, a, cls)
 
     @staticmethod
     def _synapse_records(s, types_syn):
@@ -989,7 +1087,7 @@ class Network:
 
     # -- store / restore (Network.store/restore, brian2/core/network.py:600-740) ------
     def _mutable(self):
-        """Device tensors that a run changes, by (object, attribute)."""
+        """Device tensors that a run changes, by (object, attribute) or (dict, key)."""
         out = [(self, "clock"), (self, "status")]
         for g in self.groups:
             out += [(g, "ring_spk"), (g, "ring_cnt")]
@@ -1001,6 +1099,8 @@ class Network:
                 out += [(m, "mon_n"), (m, "count_dev")]
         for s in self.paths:
             out.append((s, "events_dev"))
+            for n in sorted(s.plastic):
+                out.append((s.csr["syn"], n))
             if getattr(s, "syn_count_dev", None) is not None:
                 out.append((s, "syn_count_dev"))
         return out
@@ -1014,7 +1114,7 @@ class Network:
             self._build()
         snap = {"t_step": self.t_step, "tensors": {}, "state": {}, "monitors": {}}
         for o, a in self._mutable():
-            snap["tensors"][(id(o), a)] = getattr(o, a).clone()
+            snap["tensors"][(id(o), a)] = (o[a] if isinstance(o, dict) else getattr(o, a)).clone()
         for g in self.groups:
             if isinstance(g, NeuronGroup):
                 snap["state"][id(g)] = {n: t.clone() for n, t in g._dev.items()}
@@ -1035,7 +1135,7 @@ class Network:
         """Returns the network to the state kept by store(name)."""
         snap = self._stored[name]
         for o, a in self._mutable():
-            getattr(o, a).copy_(snap["tensors"][(id(o), a)])
+            (o[a] if isinstance(o, dict) else getattr(o, a)).copy_(snap["tensors"][(id(o), a)])
         for g in self.groups:
             if isinstance(g, NeuronGroup):
                 for n, t in g._dev.items():
@@ -1178,6 +1278,8 @@ class Network:
                        "list holds); build the Network with a larger bucket_mem_frac")
         if st & 2:
             msg.append("spike monitor buffer overflow")
+        if st & 8:
+            msg.append("synapse data names a target index outside the target group")
         return "; ".join(msg) or f"device status {st}"
 
     def _ensure_monitor_room(self, g, m, gs):

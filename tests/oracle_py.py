@@ -399,3 +399,29 @@ def load_golden(name):
     z = np.load(os.path.join(REPO, "tests", "golden", f"golden_{name}.npz"))
     meta = eval(str(z["meta"]), {"__builtins__": {}}, {"True": True, "False": False})
     return z, meta
+
+
+# ---------------------------------------------------------------------------
+# STDP (examples/synapses/STDP.py, after Song, Miller & Abbott 2000): Poisson
+# inputs (SpikeGeneratorGroup trains from the generator) -> conductance-based
+# neurons, all-to-all synapses with additive STDP (event-driven traces)
+STDP = dict(n_in=1000, n_out=20, n_steps=10000, rate=15.0, taum=10e-3, taupre=20e-3, taupost=20e-3, Ee=0.0,
+            vt=-54e-3, vr=-60e-3, El=-74e-3, taue=5e-3, gmax=0.01, dApre=0.01)
+S_STDP_W, S_STDP_V0, S_STDP_IN = 18, 19, 20
+
+
+def stdp_inputs(seed=1234, c=STDP):
+    """Inputs of the STDP configuration: trains (input i spikes at step k iff
+    U(S_STDP_IN, i*n_steps + k) < rate*dt, sorted by (step, index)); synapses
+    all-to-all in (pre, post) order; w = gmax U(S_STDP_W, k) per synapse k;
+    v0 = vr + (vt - vr) U(S_STDP_V0, j)."""
+    dt = 1e-4
+    n_in, n_out, T = c["n_in"], c["n_out"], c["n_steps"]
+    u = uniform(seed, S_STDP_IN, n_in * T).reshape(n_in, T)
+    ii, kk = np.nonzero(u < c["rate"] * dt)
+    o = np.lexsort((ii, kk))
+    pre = np.repeat(np.arange(n_in, dtype=np.int32), n_out)
+    post = np.tile(np.arange(n_out, dtype=np.int32), n_in)
+    w = c["gmax"] * uniform(seed, S_STDP_W, len(pre))
+    v0 = c["vr"] + (c["vt"] - c["vr"]) * uniform(seed, S_STDP_V0, n_out)
+    return dict(idx=ii[o].astype(np.int32), steps=kk[o].astype(np.int32), pre=pre, post=post, w=w, v0=v0)
