@@ -1063,4 +1063,67 @@ class PostPathKernel:
         self.source = "\n".join(k) + "\n"
 
 
-__all__ = ["CodeSpec", "NeuronKernel", "GatherKernel", "PostPathKernel", "scalar_values", "identifiers", "c_literal"]
+class SummedKernel:
+    """A summed variable (synapses.py:1146-1200, SummedVariableUpdater; template
+    summed_variable.pyx): target[j] = sum over the synapses of j, in synapse order,
+    of the expression, starting from 0.  One thread per target, the synapses of the
+    target (CSR by target, synapse order) folded sequentially, as the template's
+    loop over all synapses does for each target's subsequence."""
+
+    BLOCK = 256
+
+    def __init__(self, name, statements, syn_arrays, pre_reads, post_reads, var_types, target_t, scalars,
+                 float_t):
+        self.kernel_name = f"summed_{name}"
+        ps = ParamStruct(f"hbs_{name}")
+        ps.ptr("clock", "long long", const=True)
+        ps.val("step_off", "long long")
+        ps.val("dt", "double")
+        ps.val("N", "int")
+        ps.val("pad", "int")
+        ps.ptr("post_ptr", "unsigned int", const=True)
+        ps.ptr("post_syn", "unsigned int", const=True)
+        ps.ptr("pre_of", "int", const=True)
+        ps.ptr("target", target_t)
+        for v, ct in syn_arrays.items():
+            ps.ptr(f"s_{v}", ct, const=True)
+        for n, gv in sorted(pre_reads.items()):
+            ps.ptr(f"a_{n}", var_types[n], const=True)
+        for n, gv in sorted(post_reads.items()):
+            ps.ptr(f"a_{n}", var_types[n], const=True)
+        for n, (_, ct) in sorted(scalars.items()):
+            ps.val(f"c_{n}", ct)
+        self.struct = ps
+        types = dict(syn_arrays)
+        types.update({n: var_types[n] for n in list(pre_reads) + list(post_reads)})
+        types.update({"_idx": "int", "_presynaptic_idx": "int", "_postsynaptic_idx": "int", "t": "double"})
+        for n, (_, ct) in scalars.items():
+            types[n] = ct
+        lines = _rename(translate_block(statements, {}, types, set(), float_t), {n: f"p.c_{n}" for n in scalars})
+        B = self.BLOCK
+        k = [source_header(B), ps.c_decl()]
+        k.append(f'extern "C" __global__ void __launch_bounds__({B}) {self.kernel_name}(const {ps.name} p) {{')
+        k += [
+            "    const long long step = p.clock[0] + p.step_off;",
+            "    const double t = (double)step * p.dt;",
+            "    (void)t;",
+            f"    const int j = blockIdx.x * {B} + threadIdx.x;",
+            "    if (j >= p.N) return;",
+            "    const int _postsynaptic_idx = j;",
+            f"    {target_t} hb_acc = ({target_t})0;",
+            "    for (unsigned int k_ = p.post_ptr[j]; k_ < p.post_ptr[j + 1]; k_++) {",
+            "        const unsigned int e = p.post_syn[k_];",
+            "        const int _idx = (int)e;",
+            "        const int _presynaptic_idx = p.pre_of[e];",
+            "        (void)_idx; (void)_presynaptic_idx; (void)_postsynaptic_idx;",
+        ]
+        k += [f"        const {ct} {v} = p.s_{v}[e];" for v, ct in syn_arrays.items()]
+        k += [f"        const {var_types[n]} {n} = p.a_{n}[_presynaptic_idx];" for n in sorted(pre_reads)]
+        k += [f"        const {var_types[n]} {n} = p.a_{n}[_postsynaptic_idx];" for n in sorted(post_reads)]
+        k += ["        " + x for x in lines]
+        k += ["        hb_acc += _synaptic_var;", "    }", "    p.target[j] = hb_acc;", "}"]
+        self.source = "\n".join(k) + "\n"
+
+
+__all__ = ["CodeSpec", "NeuronKernel", "GatherKernel", "PostPathKernel", "SummedKernel", "scalar_values",
+           "identifiers", "c_literal"]

@@ -25,6 +25,7 @@ from .kernels import (
     GatherKernel,
     NeuronKernel,
     PostPathKernel,
+    SummedKernel,
     fold_capacity,
     identifiers,
     rec_words,
@@ -124,6 +125,7 @@ class Model:
         self.name = d.get("model", "model")
         self.group = {k: CodeSpec(v) for k, v in d["group"].items()}
         self.pathways = {k: CodeSpec(v) for k, v in d.get("pathways", {}).items()}
+        self.summed = {k: CodeSpec(v) for k, v in d.get("summed", {}).items()}
 
     @classmethod
     def load(cls, name):
@@ -293,14 +295,15 @@ class Synapses:
     ``storage`` may store a float64 synaptic variable as float32 on the device
     when every value is exactly representable (checked)."""
 
-    def __init__(self, source, target, model, name=None, storage=None, order=0, count_events=False, on_post=None,
-                 post_delay=0.0):
+    def __init__(self, source, target, model=None, name=None, storage=None, order=0, count_events=False, on_post=None,
+                 post_delay=0.0, summed=None):
         self.name = name or _unique("synapses")
         self.count_events = count_events
         self.pathway_name = f"{self.name}_pre"
         self.order = order
         self.post_spec = on_post
         self.post_delay = float(post_delay)
+        self.summed = dict(summed or {})  # {target variable: summed_variable code object}
         if isinstance(source, Subgroup):
             self.source, self.src_range = source.group, (source.start, source.stop)
         else:
@@ -347,6 +350,33 @@ class Synapses:
         return out
 
     def _analyse(self):
+        self.plastic = set()
+        self.summed_reads = {}  # {summed target: ({pre name: group var}, {post name: group var})}
+        used = set()
+        for tv, sp in self.summed.items():
+            if tv not in self.target.vars:
+                raise ValueError(f"summed variable target {tv!r} is not a variable of {self.target.name}")
+            pre_r, post_r = {}, {}
+            for n in identifiers(sp.vector):
+                v = sp.variables.get(n)
+                if v is None or v["kind"] in ("Constant", "Function", "AuxiliaryVariable") or n in ("t", "dt"):
+                    continue
+                ix = sp.indices.get(n, "_idx")
+                if ix == "_presynaptic_idx" and n.endswith("_pre"):
+                    pre_r[n] = n[:-4]
+                elif ix == "_postsynaptic_idx" and n.endswith("_post"):
+                    post_r[n] = n[:-5]
+                elif ix == "_idx" and n not in ("_presynaptic_idx", "_postsynaptic_idx"):
+                    used.add(n)
+                elif n not in ("_presynaptic_idx", "_postsynaptic_idx"):
+                    raise NotImplementedError(f"summed variable code reads {n!r}")
+            self.summed_reads[tv] = (pre_r, post_r)
+        if self.spec is None:
+            if self.post_spec is not None:
+                raise NotImplementedError("an on_post pathway without an on_pre pathway")
+            self.target_var = self.op = None
+            self.syn_vars = sorted(used)
+            return
         vs = self.spec.vector
         if not vs:
             raise NotImplementedError("empty on_pre code")
@@ -359,7 +389,7 @@ class Synapses:
         self.op = vs[tgt[0]]["op"]
         # synaptic variables written by either pathway (plasticity) are read and written in place
         self.plastic = self._synaptic_writes(self.spec, self.target_index, "on_pre")
-        used = self._synaptic_vars(self.spec, {self.target_var}, "on_pre")
+        used |= self._synaptic_vars(self.spec, {self.target_var}, "on_pre")
         if self.post_spec is not None:
             if any(self.post_spec.indices.get(st["var"], "_idx") != "_idx" for st in self.post_spec.vector):
                 raise NotImplementedError("on_post code may update synaptic variables only")
@@ -388,8 +418,8 @@ class Synapses:
     def connect_device_csr(self, seg_ptr, post, syn, dmin, n_seg, n_syn):
         """Connectivity built on the device (see synth.py): CSR by (source, delay);
         synapse index = storage position."""
-        if self.post_spec is not None:
-            raise NotImplementedError("on_post pathways need connect(i, j)")
+        if self.post_spec is not None or self.summed:
+            raise NotImplementedError("on_post pathways and summed variables need connect(i, j)")
         self.csr = dict(seg_ptr=seg_ptr, post=post, syn=syn, dmin=int(dmin), n_seg=int(n_seg), n_syn=int(n_syn))
 
     def get(self, name):
@@ -444,15 +474,17 @@ class Synapses:
                 syn[n] = dev(v)
         self.csr = dict(seg_ptr=dev(seg_ptr.astype(np.uint32).view(np.int32)), post=dev(self.post[order]),
                         syn=syn, dmin=dmin, n_seg=n_seg, n_syn=S, order=order)
-        if self.post_spec is not None:
-            # CSR by target for the on_post pathway: storage positions of each target's
-            # synapses in synapse order (the post pathway's queue order)
+        if self.post_spec is not None or self.summed:
+            # CSR by target for the on_post pathway and summed variables: storage positions
+            # of each target's synapses in synapse order (the post pathway's queue order,
+            # summed_variable.pyx's loop order)
             pos_post = self.post[order].astype(np.int64)
             perm = np.lexsort((order, pos_post))
             ptr_ = np.zeros(self.target.N + 1, np.uint64)
             np.cumsum(np.bincount(pos_post, minlength=self.target.N), out=ptr_[1:])
             self.csr["post_ptr"] = dev(ptr_.astype(np.uint32).view(np.int32))
             self.csr["post_syn"] = dev(perm.astype(np.int32))
+            self.csr["pre_of"] = dev(self.pre[order].astype(np.int32))
 
     @property
     def n_events(self):
@@ -649,14 +681,16 @@ class Network:
         torch = _torch()
         D = self.device = Device.get()
         groups = [o for o in self.objs if isinstance(o, (NeuronGroup, SpikeGeneratorGroup, PoissonGroup))]
-        paths = sorted([o for o in self.objs if isinstance(o, Synapses)], key=lambda s: (s.order, s.pathway_name))
+        all_syn = sorted([o for o in self.objs if isinstance(o, Synapses)], key=lambda s: (s.order, s.pathway_name))
+        paths = [s for s in all_syn if s.spec is not None]  # synapses with an on_pre pathway
+        self.summed_syn = [s for s in all_syn if s.summed]
         smons = [o for o in self.objs if isinstance(o, SpikeMonitor)]
         stmons = [o for o in self.objs if isinstance(o, StateMonitor)]
         self.rmons = [o for o in self.objs if isinstance(o, PopulationRateMonitor)]
         for m in smons + stmons + self.rmons:
             if m.source not in groups:
                 raise RuntimeError(f"{m.source.name} recorded by {m.name} is not in the network")
-        for s in paths:
+        for s in all_syn:
             for g in (s.source, s.target):
                 if g not in groups:
                     raise RuntimeError(f"{g.name} used by {s.name} is not in the network")
@@ -664,7 +698,7 @@ class Network:
         self.clock = torch.zeros(1, dtype=torch.int64, device="cuda")
         self.clock.fill_(self.t_step)
         self.status = torch.zeros(1, dtype=torch.int32, device="cuda")
-        for s in paths:
+        for s in all_syn:
             s._build_csr(self.dt)
             s.post_delay_steps = int(s.post_delay / self.dt + 0.5)  # spikequeue.h:90
         # per group: device state, spike ring
@@ -698,6 +732,10 @@ class Network:
         self._plan = ctypes.c_void_p()
         _lib.check(L.hb_plan_create(D.ctx, ptr(self.clock), ctypes.byref(self._plan)), "hb_plan_create")
         self._launches = []
+        # summed variables first: their updaters run in the 'groups' slot before the target's
+        # state update (synapses.py:146-160, order target.order - 1)
+        for s in self.summed_syn:
+            self._build_summed(s)
         for g in groups:
             rm = [m for m in self.rmons if m.source is g]
             if len(rm) > 1:
@@ -1030,6 +1068,52 @@ class Network:
         s.launch_index = len(self._launches)
         s.args, s.grid = a, grid
         self._add_launch(s.fn, grid, K.block, a, cls)
+
+    def _build_summed(self, s):
+        g = s.target
+        for tv, (pre_r, post_r) in s.summed_reads.items():
+            for grp, names in ((s.source, pre_r), (g, post_r)):
+                for gv in names.values():
+                    if not isinstance(grp, NeuronGroup) or gv not in grp.vars:
+                        raise ValueError(f"summed variable of {s.name} reads {gv!r}, not a variable of {grp.name}")
+                    # the fold of a step's events and deferred resets happen at the start of the next
+                    # group update: a summed variable computed before it would read stale values
+                    if any(p.target is grp and p.target_var == gv for p in self.paths) or (
+                            self._reset_mode(grp) == "deferred"):
+                        raise NotImplementedError(f"summed variable reading {gv!r}, which synapses or a deferred "
+                                                  "reset also update")
+            if any(p.target is g and p.target_var == tv for p in self.paths):
+                raise NotImplementedError(f"{tv!r} is both a summed variable and the target of a pathway")
+        s.summed_kernels = []
+        types_syn = s.syn_ctypes()
+        for tv, sp in sorted(s.summed.items()):
+            pre_r, post_r = s.summed_reads[tv]
+            var_types = {n: s.source.vars[gv] for n, gv in pre_r.items()}
+            var_types.update({n: g.vars[gv] for n, gv in post_r.items()})
+            used = {n: ct for n, ct in types_syn.items() if n in identifiers(sp.vector)}
+            host = {"dt": self.dt}
+            host.update(getattr(s, "namespace", {}) or {})
+            sc = scalar_values(sp, host, g.float_t)
+            K = SummedKernel(f"{s.name}_{tv}", sp.vector, used, pre_r, post_r, var_types, g.vars[tv], sc, g.float_t)
+            fn = self.device.function(K.source, K.kernel_name)
+            cls = K.struct.ctypes_struct()
+            v = {"clock": self.clock.data_ptr(), "step_off": 0, "dt": self.dt, "N": g.N,
+                 "post_ptr": s.csr["post_ptr"].data_ptr(), "post_syn": s.csr["post_syn"].data_ptr(),
+                 "pre_of": s.csr["pre_of"].data_ptr(), "target": g._dev[tv].data_ptr()}
+            for n in used:
+                v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
+            for n, gv in pre_r.items():
+                v[f"a_{n}"] = s.source._dev[gv].data_ptr()
+            for n, gv in post_r.items():
+                v[f"a_{n}"] = g._dev[gv].data_ptr()
+            for n, (val, ct) in sc.items():
+                v[f"c_{n}"] = val
+            a = cls()
+            for name, _ in cls._fields_:
+                if name in v:
+                    setattr(a, name, v[name])
+            s.summed_kernels.append(K)
+            self._add_launch(fn, (g.N + K.BLOCK - 1) // K.BLOCK, K.BLOCK, a, cls)
 
     def _build_post(self, s):
         host = {"dt": self.dt}

@@ -62,6 +62,7 @@ HOT_TEMPLATES = {
     "spikemonitor",
     "statemonitor",
     "spikegenerator",
+    "summed_variable",
 }
 
 CAPTURED = []

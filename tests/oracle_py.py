@@ -425,3 +425,20 @@ def stdp_inputs(seed=1234, c=STDP):
     w = c["gmax"] * uniform(seed, S_STDP_W, len(pre))
     v0 = c["vr"] + (c["vt"] - c["vr"]) * uniform(seed, S_STDP_V0, n_out)
     return dict(idx=ii[o].astype(np.int32), steps=kk[o].astype(np.int32), pre=pre, post=post, w=w, v0=v0)
+
+
+# ---------------------------------------------------------------------------
+# gap junctions (examples/synapses/gapjunctions.py): a summed variable
+# Igap_post = w * (v_pre - v_post) over all-to-all synapses (self-pairs included)
+GAP = dict(n=300, n_steps=5000, v0=1.05, tau=10e-3, w=0.02)
+S_GAP_W, S_GAP_V = 21, 22
+
+
+def gap_inputs(seed=1234, c=GAP):
+    """Synapses in (pre, post) order; w = w (0.5 + U(S_GAP_W, k)); v = U(S_GAP_V, i)."""
+    n = c["n"]
+    pre = np.repeat(np.arange(n, dtype=np.int32), n)
+    post = np.tile(np.arange(n, dtype=np.int32), n)
+    w = c["w"] * (0.5 + uniform(seed, S_GAP_W, n * n))
+    v = uniform(seed, S_GAP_V, n)
+    return dict(pre=pre, post=post, w=w, v=v)
