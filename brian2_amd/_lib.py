@@ -50,6 +50,7 @@ _SIGS = {
     "hb_module_unload": ([_vp], _i),
     "hb_module_function": ([_vp, _cp, _P(_vp)], _i),
     "hb_launch": ([_vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _vp, _vp, _sz], _i),
+    "hb_function_info": ([_vp, _P(_i), _P(_i64), _P(_i64)], _i),
     "hb_plan_create": ([_vp, _vp, _P(_vp)], _i),
     "hb_plan_add": ([_vp, _vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _vp, _sz, ctypes.c_long], _i),
     "hb_plan_set_args": ([_vp, _i, _vp, _sz], _i),

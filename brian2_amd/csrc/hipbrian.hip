@@ -224,6 +224,19 @@ extern "C" int hb_module_function(hb_module* m, const char* name, void** fn) {
     return 0;
 }
 
+extern "C" int hb_function_info(void* fn, int* vgprs, int64_t* scratch_bytes, int64_t* lds_bytes) {
+    const hb_fn* f = (const hb_fn*)fn;
+    if (!f || !f->mfn) return set_err("hb_function_info: not a generated kernel");
+    int regs = 0, local = 0, shared = 0;
+    HB_HIP(hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, f->mfn));
+    HB_HIP(hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f->mfn));
+    HB_HIP(hipFuncGetAttribute(&shared, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, f->mfn));
+    *vgprs = regs;
+    *scratch_bytes = local;
+    *lds_bytes = shared;
+    return 0;
+}
+
 extern "C" int hb_launch(void* fn, unsigned grid, unsigned block, unsigned shmem, void* stream, const void* args,
                          size_t argsize) {
     if (grid == 0) return 0;

@@ -116,7 +116,7 @@ def _rename(lines, names):
     return [pat.sub(lambda m: names[m.group(1)], ln) for ln in lines]
 
 
-FOLD_EVENTS_PER_THREAD = 8    # fold: events a thread keeps in registers
+FOLD_EVENTS_PER_THREAD = 2    # fold: events a thread keeps in registers
 GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per window
 GATHER_STAGE_RUNS = True      # gather: sort a window's events by tile in LDS before writing them
 LDS_BYTES = 163840            # LDS of one CU
@@ -157,12 +157,12 @@ def fold_lds_bytes(T, cap, val_t, npt=1):
     return cap * (8 + vbytes + 2) + 8 * T * npt + 1024
 
 
-def fold_capacity(T, val_t, npt=1):
-    """Events of one tile sorted in LDS at a time.  A step kernel needs about 120
-    VGPRs, i.e. 16 waves per CU, so max(1, 1024 // T) workgroups of T threads
-    share the LDS of a CU; each gets an equal share."""
+def fold_capacity(T, val_t, npt=1, wgs_per_cu=None):
+    """Events of one tile sorted in LDS at a time: the workgroups that share a CU
+    (by default max(1, 1024 // T), i.e. 16 waves of up to 128 VGPRs) get equal
+    shares of its LDS."""
     vbytes = 8 if val_t in ("double", "long long") else 4
-    per_wg = LDS_BYTES // max(1, 1024 // T)
+    per_wg = LDS_BYTES // (wgs_per_cu or max(1, 1024 // T))
     cap = (per_wg - 8 * T * npt - 1024) // (8 + vbytes + 2)
     return int(max(128, cap - cap % 64))
 
@@ -368,7 +368,7 @@ class NeuronKernel:
     def __init__(self, name, N, arrays, float_t, stateupdate=None, threshold=None, reset=None,
                  reset_mode="none", scalars=None, fold=None, lds_cap=0, val_t="double",
                  spikemon=None, statemon=None, flush=False, uses_refractory=False, statemon_when="start",
-                 tile=256, npt=1, ratemon=None):
+                 tile=256, npt=1, ratemon=None, waves_per_eu=0):
         self.name = name
         self.N = N
         self.arrays = arrays          # {var: ctype} of the group's state arrays
@@ -383,6 +383,7 @@ class NeuronKernel:
             raise ValueError("fold capacity must be an even number of events, at least 128, whose LDS fits a CU")
         self.spikemon = spikemon      # (start, stop) or None
         self.ratemon = None if flush else ratemon  # (start, stop) of a PopulationRateMonitor or None
+        self.waves_per_eu = int(waves_per_eu)  # > 0: ask for that many waves per SIMD (at most 512/w VGPRs)
         self.statemon = statemon      # [vars] or None
         self.statemon_when = statemon_when
         self.flush = flush
@@ -460,7 +461,8 @@ class NeuronKernel:
         k = [source_header(T), ps.c_decl()]
         kname = f"{'flush' if self.flush else 'step'}_{self.name}"
         self.kernel_name = kname
-        k.append(f'extern "C" __global__ void __launch_bounds__({T}) {kname}(const {ps.name} p) {{')
+        occ = f" __attribute__((amdgpu_waves_per_eu({self.waves_per_eu})))" if self.waves_per_eu else ""
+        k.append(f'extern "C" __global__ void __launch_bounds__({T}){occ} {kname}(const {ps.name} p) {{')
         k.append("    const long long step = p.clock[0] + p.step_off;")
         k.append("    const double t = (double)step * p.dt;")
         k.append("    const double dt = p.dt;")

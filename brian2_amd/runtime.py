@@ -97,6 +97,12 @@ class Device:
         _lib.check(L.hb_module_function(mod, name.encode(), ctypes.byref(fn)), f"function {name}")
         return fn
 
+    def function_info(self, fn):
+        """(VGPRs per thread, spill bytes per thread, static LDS bytes) of a generated kernel."""
+        r, sc, lds = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(L.hb_function_info(fn, ctypes.byref(r), ctypes.byref(sc), ctypes.byref(lds)), "hb_function_info")
+        return r.value, sc.value, lds.value
+
     def sync(self):
         _torch().cuda.synchronize()
 
@@ -782,8 +788,11 @@ class Network:
         T = g.tile
         g.n_tiles = (g.N + T - 1) // T
         vbytes = 8 if g.val_t in ("double", "long long") else 4
-        g.lds_cap = int(min(self.bucket_cap, fold_capacity(g.block, g.val_t, g.npt)) if self.bucket_cap
-                        else fold_capacity(g.block, g.val_t, g.npt))
+        # tiles of 1024 neurons, one per thread: two workgroups per CU when the step kernel fits
+        # in 64 VGPRs (checked when it is built), so each gets half the LDS
+        g.two_per_cu = g.block == 1024 and g.npt == 1
+        cap = fold_capacity(g.block, g.val_t, g.npt, wgs_per_cu=2 if g.two_per_cu else None)
+        g.lds_cap = int(min(self.bucket_cap, cap) if self.bucket_cap else cap)
         window = _kernels.GATHER_BLOCK * _kernels.GATHER_EVENTS_PER_THREAD
         total = sum(int(s.csr["n_syn"]) for s in incoming)
         for s in incoming:
@@ -882,15 +891,25 @@ class Network:
             for v in st.variables:
                 if v not in g.vars:
                     raise KeyError(f"StateMonitor variable {v!r} is not a state variable of {g.name}")
-        K = NeuronKernel(g.name, g.N, g.vars, g.float_t, g.su, g.th, g.rs, mode, scalars, fold,
-                         getattr(g, "lds_cap", 0), getattr(g, "val_t", "double"),
-                         spikemon=g.smon.range if g.smon else None,
-                         statemon=g.stmon.variables if g.stmon else None,
-                         uses_refractory=g.uses_refractory,
-                         statemon_when=g.stmon.when if g.stmon else "start", tile=g.block, npt=g.npt,
-                         ratemon=g.rmon.range if g.rmon else None)
-        g.kernel = K
+        def step_kernel(waves):
+            return NeuronKernel(g.name, g.N, g.vars, g.float_t, g.su, g.th, g.rs, mode, scalars, fold,
+                                getattr(g, "lds_cap", 0), getattr(g, "val_t", "double"),
+                                spikemon=g.smon.range if g.smon else None,
+                                statemon=g.stmon.variables if g.stmon else None,
+                                uses_refractory=g.uses_refractory,
+                                statemon_when=g.stmon.when if g.stmon else "start", tile=g.block, npt=g.npt,
+                                ratemon=g.rmon.range if g.rmon else None, waves_per_eu=waves)
+
+        # two workgroups of 1024 threads per CU need at most 64 VGPRs: keep that build only
+        # if the compiler fits it without spilling
+        g.waves = 8 if getattr(g, "two_per_cu", False) else 0
+        K = step_kernel(g.waves)
         g.fn = D.function(K.source, K.kernel_name)
+        if g.waves and D.function_info(g.fn)[1] > 0:
+            g.waves = 0
+            K = step_kernel(0)
+            g.fn = D.function(K.source, K.kernel_name)
+        g.kernel = K
         g.struct_cls = K.struct.ctypes_struct()
         if g.smon:
             self._alloc_spike_monitor(g, g.smon)
@@ -907,7 +926,7 @@ class Network:
                              getattr(g, "lds_cap", 0), getattr(g, "val_t", "double"), flush=True,
                              uses_refractory=g.uses_refractory,
                              statemon=g.stmon.variables if end_mon else None, statemon_when="end", tile=g.block,
-                             npt=g.npt)
+                             npt=g.npt, waves_per_eu=g.waves)
             g.flush_fn = D.function(F.source, F.kernel_name)
             g.flush_struct_cls = F.struct.ctypes_struct()
         else:

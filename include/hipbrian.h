@@ -66,6 +66,11 @@ int hb_module_load(hb_ctx* ctx, const char* source, const char* extra_options, h
                    size_t loglen);
 int hb_module_unload(hb_module* mod);
 int hb_module_function(hb_module* mod, const char* name, void** fn);
+/* Resources of a generated kernel: registers per thread, private (spill) memory
+ * per thread, static LDS per workgroup.  No counterpart in Brian (the cython
+ * runtime compiles once per code object); the runtime uses it to pick between a
+ * two-workgroups-per-CU build of a step kernel and a one-workgroup build. */
+int hb_function_info(void* fn, int* vgprs, int64_t* scratch_bytes, int64_t* lds_bytes);
 /* Launches a module function with a packed argument buffer (the kernel takes
  * one struct by value whose bytes are `args`). */
 int hb_launch(void* fn, unsigned grid, unsigned block, unsigned shmem, void* stream, const void* args,
