@@ -143,3 +143,49 @@ def test_gather_kernel_limits():
         GatherKernel("p", 16, 1, 0, spec.vector, {}, sc, "double", "double", "ge", 4, 256)
     with pytest.raises(NotImplementedError):
         GatherKernel("p", 0, 5000, 0, spec.vector, {}, sc, "double", "double", "ge", 4, 256)
+
+
+def test_plasticity_and_summed_kernels_compile_for_gfx950():
+    """The STDP pathway (on_pre with synaptic writes, on_post) and the gap-junction
+    summed variable generate kernels that compile, without floating-point atomics,
+    and the synapses' analysis finds the target update and the written variables."""
+    from brian2_amd.kernels import PostPathKernel, SummedKernel
+    from brian2_amd.runtime import SpikeGeneratorGroup, Synapses
+
+    m = Model.load("stdp")
+    G = NeuronGroup(20, m)
+    S = Synapses(SpikeGeneratorGroup(10, [0], timebins=[0]), G, m.pathways["pre"], on_post=m.pathways["post"])
+    assert S.target_var == "ge" and S.op == "+=" and S.spec.vector[S.target_index]["var"] == "ge"
+    assert S.plastic == {"Apre", "Apost", "w", "lastupdate"}
+    types = {n: "double" for n in S.syn_vars}
+    gk = GatherKernel("stdp", 0, 1, 0, S.spec.vector, types, scalar_values(S.spec, {"dt": 1e-4}, "double"),
+                      "double", "double", "ge", 1, 256, target_index=S.target_index, plastic=S.plastic)
+    assert gk.rec_arrays == {} and gk.rec_words == 1  # plastic variables are not copied into records
+    pk = PostPathKernel("stdp", S.post_spec.vector, types, S.plastic,
+                        scalar_values(S.post_spec, {"dt": 1e-4}, "double"), "double", 0)
+    g = Model.load("gap")
+    H = NeuronGroup(30, g)
+    T = Synapses(H, H, summed={"Igap": g.summed["Igap"]})
+    pre_r, post_r = T.summed_reads["Igap"]
+    assert T.syn_vars == ["w"] and pre_r == {"v_pre": "v"} and post_r == {"v_post": "v"}
+    sk = SummedKernel("gap_Igap", g.summed["Igap"].vector, {"w": "double"}, pre_r, post_r,
+                      {"v_pre": "double", "v_post": "double"}, "double", {}, "double")
+    for src in (gk.source, pk.source, sk.source):
+        _no_float_atomics(src)
+        _compile(src)
+
+
+def test_step_kernel_two_workgroups_per_cu_build():
+    """The 1024-thread step kernel of the 1M-neuron model asks for 8 waves per SIMD and
+    takes the LDS share of two workgroups per CU."""
+    from brian2_amd.kernels import LDS_BYTES, fold_capacity, fold_lds_bytes
+
+    m = Model.load("cuba_synw")
+    G = NeuronGroup(1000, m, float_t="double")
+    cap = fold_capacity(1024, "float", 1, wgs_per_cu=2)
+    assert fold_lds_bytes(1024, cap, "float") <= LDS_BYTES // 2
+    K = NeuronKernel("g", 1000, G.vars, "double", G.su, G.th, G.rs, "fused", _scalars(G),
+                     [(0, "gi", "+="), (1, "ge", "+=")], cap, "float", spikemon=(0, 1000), tile=1024,
+                     waves_per_eu=8)
+    assert "amdgpu_waves_per_eu(8)" in K.source
+    _compile(K.source)
