@@ -71,10 +71,13 @@ def kernel_bytes(kind, n, events, spikes, fbytes):
     """Algorithmic bytes of one launch: neuron state read + write (v, ge, gi),
     lastspike read, not_refractory read + write; per spike the ring entry, the
     monitor record and the lastspike write; per event the post index, the fp32
-    weight and the delay code."""
+    weight and the delay code.  A fused launch (group step and the gathers into
+    the group) moves both."""
+    step = n * (3 * fbytes * 2 + 8 + 2) + spikes * (4 + 8 + 8)
     if kind == "step":
-        return n * (3 * fbytes * 2 + 8 + 2) + spikes * (4 + 8 + 8)
-    return events * (4 + 4 + 1)
+        return step
+    gather = events * (4 + 4 + 1)
+    return step + gather if kind == "fused" else gather
 
 
 def sweep_bench(n_nets=1024, steps=10000, sample=16, sample_steps=1000, float_t="double"):
@@ -241,7 +244,9 @@ def main():
     kernels = []
     for kind, name, ms in prof:
         avg_us = ms / args.steps * 1e3
-        alg = kernel_bytes(kind, args.N, ev_p.get(name, 0) / args.steps, spikes / args.steps, fbytes)
+        # a fused launch runs the group's step and the gathers of every pathway into it
+        ev = sum(ev_p[s.name] for s in net.paths if s.target.name == name) if kind == "fused" else ev_p.get(name, 0)
+        alg = kernel_bytes(kind, args.N, ev / args.steps, spikes / args.steps, fbytes)
         kernels.append(dict(kernel=f"{kind}_{name}", avg_us=avg_us, alg_bytes=alg,
                             gbs=alg / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0))
     for k in kernels:

@@ -51,6 +51,8 @@ _SIGS = {
     "hb_module_function": ([_vp, _cp, _P(_vp)], _i),
     "hb_launch": ([_vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _vp, _vp, _sz], _i),
     "hb_function_info": ([_vp, _P(_i), _P(_i64), _P(_i64)], _i),
+    "hb_function_spills": ([_vp, _P(_i), _P(_i)], _i),
+    "hb_code_object_spills": ([_vp, _sz, _P(_i), _P(_i)], _i),
     "hb_plan_create": ([_vp, _vp, _P(_vp)], _i),
     "hb_plan_add": ([_vp, _vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _vp, _sz, ctypes.c_long], _i),
     "hb_plan_set_args": ([_vp, _i, _vp, _sz], _i),

@@ -482,26 +482,18 @@ __device__ __forceinline__ int hb_block_alloc(int v, int* ctr) {
     return __builtin_amdgcn_readlane(base, 63) + x - v;
 }
 // exclusive scan of one int per thread over a workgroup of HB_T threads (HB_T/64 waves):
-// scan within each wave, wave totals through LDS, each thread adds the totals of the
-// waves before its own (sh: HB_T/64 ints, 16-byte aligned)
+// scan within each wave, wave totals through LDS; lane l of every wave reads wave l's
+// total and a second wave scan gives the totals before this wave (lane wid - 1) and the
+// workgroup's total (lane 63), read out as wave-uniform values (sh: HB_T/64 ints)
 __device__ __forceinline__ int hb_block_exscan(int v, int* sh, int* total) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int x = hb_wave_incl_scan(v);
     if (lane == 63) sh[wid] = x;
     __syncthreads();
-    int pre = 0, tot = 0;
-#if HB_T % 256 == 0
-    #pragma unroll
-    for (int w = 0; w < HB_T / 64; w += 4) {
-        const int4 t = *(const int4*)(sh + w);
-        pre += (w < wid ? t.x : 0) + (w + 1 < wid ? t.y : 0) + (w + 2 < wid ? t.z : 0) + (w + 3 < wid ? t.w : 0);
-        tot += t.x + t.y + t.z + t.w;
-    }
-#else
-    #pragma unroll
-    for (int w = 0; w < HB_T / 64; w++) { const int t = sh[w]; pre += w < wid ? t : 0; tot += t; }
-#endif
-    *total = tot;
+    const int w = hb_wave_incl_scan(lane < HB_T / 64 ? sh[lane] : 0);
+    const int pre = wid > 0 ? __builtin_amdgcn_readlane(w, wid - 1) : 0;
+    *total = __builtin_amdgcn_readlane(w, 63);
     __syncthreads();
     return x - v + pre;
 }

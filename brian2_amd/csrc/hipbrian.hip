@@ -137,9 +137,11 @@ extern "C" int hb_stream_sync(void* stream) {
 // ---------------------------------------------------------------------------
 // functions: generated (module) or built in
 // ---------------------------------------------------------------------------
+struct hb_module;
 struct hb_fn {
     hipFunction_t mfn;        // module function (hiprtc)
     const void* kernel;       // built-in __global__ taking one struct argument
+    const hb_module* mod;     // module of mfn
 };
 
 static int launch_fn(const hb_fn* f, unsigned grid, unsigned block, unsigned shmem, hipStream_t s,
@@ -160,7 +162,38 @@ struct hb_module {
     hb_ctx* ctx;
     hipModule_t mod;
     std::vector<std::unique_ptr<hb_fn>> fns;
+    int vgpr_spills = -1, sgpr_spills = -1;  // from the code object's metadata (-1: not found)
 };
+
+// The largest value of a msgpack map key `key` (a fixstr) followed by an unsigned integer,
+// over every occurrence in [p, p + n); -1 if there is none.  The kernels' metadata in an
+// AMDGPU code object (the NT_AMDGPU_METADATA note) is a msgpack document with one map per
+// kernel.
+static long long msgpack_uint_after(const unsigned char* p, size_t n, const char* key) {
+    const size_t k = strlen(key);
+    long long best = -1;
+    for (size_t i = 0; k < 32 && i + k + 2 <= n; i++) {
+        if (p[i] != (unsigned char)(0xa0 | k) || memcmp(p + i + 1, key, k) != 0) continue;
+        const unsigned char* v = p + i + 1 + k;
+        const size_t left = n - (i + 1 + k);
+        long long x = -1;
+        if (v[0] < 0x80) x = v[0];
+        else if (v[0] == 0xcc && left >= 2) x = v[1];
+        else if (v[0] == 0xcd && left >= 3) x = ((long long)v[1] << 8) | v[2];
+        else if (v[0] == 0xce && left >= 5)
+            x = ((long long)v[1] << 24) | ((long long)v[2] << 16) | ((long long)v[3] << 8) | v[4];
+        if (x > best) best = x;
+    }
+    return best;
+}
+
+extern "C" int hb_code_object_spills(const void* code, size_t nbytes, int* vgpr_spills, int* sgpr_spills) {
+    if (!code || !vgpr_spills || !sgpr_spills) return set_err("hb_code_object_spills: null argument");
+    const unsigned char* p = (const unsigned char*)code;
+    *vgpr_spills = (int)msgpack_uint_after(p, nbytes, ".vgpr_spill_count");
+    *sgpr_spills = (int)msgpack_uint_after(p, nbytes, ".sgpr_spill_count");
+    return 0;
+}
 
 extern "C" int hb_module_load(hb_ctx* c, const char* source, const char* extra_options, hb_module** out,
                               char* log, size_t loglen) {
@@ -203,6 +236,7 @@ extern "C" int hb_module_load(hb_ctx* c, const char* source, const char* extra_o
     hiprtcDestroyProgram(&prog);
     std::unique_ptr<hb_module> m(new hb_module());
     m->ctx = c;
+    hb_code_object_spills(code.data(), code.size(), &m->vgpr_spills, &m->sgpr_spills);
     HB_HIP(hipModuleLoadData(&m->mod, code.data()));
     *out = m.release();
     return 0;
@@ -218,6 +252,7 @@ extern "C" int hb_module_unload(hb_module* m) {
 extern "C" int hb_module_function(hb_module* m, const char* name, void** fn) {
     std::unique_ptr<hb_fn> f(new hb_fn());
     f->kernel = nullptr;
+    f->mod = m;
     HB_HIP(hipModuleGetFunction(&f->mfn, m->mod, name));
     *fn = f.get();
     m->fns.push_back(std::move(f));
@@ -234,6 +269,14 @@ extern "C" int hb_function_info(void* fn, int* vgprs, int64_t* scratch_bytes, in
     *vgprs = regs;
     *scratch_bytes = local;
     *lds_bytes = shared;
+    return 0;
+}
+
+extern "C" int hb_function_spills(void* fn, int* vgpr_spills, int* sgpr_spills) {
+    const hb_fn* f = (const hb_fn*)fn;
+    if (!f || !f->mfn || !f->mod) return set_err("hb_function_spills: not a generated kernel");
+    *vgpr_spills = f->mod->vgpr_spills;
+    *sgpr_spills = f->mod->sgpr_spills;
     return 0;
 }
 

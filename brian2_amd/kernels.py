@@ -43,6 +43,7 @@ KEY_TGT_SHIFT = 48  # event key: [63:48] target in tile, [47:44] rank, [43:32] d
 MAX_RANK = 16
 MAX_SEG = 4096
 GATHER_BLOCK = 1024
+DECL_MARK = "// ---- generated declarations ----"  # the per-kernel part of a source starts here
 
 
 class CodeSpec:
@@ -458,7 +459,7 @@ class NeuronKernel:
     def _source(self, ps):
         T, NPT = self.T, self.npt
         TN = T * NPT
-        k = [source_header(T), ps.c_decl()]
+        k = [source_header(T), DECL_MARK, ps.c_decl()]
         kname = f"{'flush' if self.flush else 'step'}_{self.name}"
         self.kernel_name = kname
         occ = f" __attribute__((amdgpu_waves_per_eu({self.waves_per_eu})))" if self.waves_per_eu else ""
@@ -669,7 +670,8 @@ class GatherKernel:
     consecutive positions)."""
 
     def __init__(self, name, rank, n_seg, dmin, on_pre, syn_arrays, scalars, val_t, float_t, target_var,
-                 n_tiles, tile, count_synapses=False, pairs=256, target_index=None, plastic=()):
+                 n_tiles, tile, count_synapses=False, pairs=256, target_index=None, plastic=(),
+                 events_per_thread=None):
         if rank >= MAX_RANK:
             raise NotImplementedError(f"more than {MAX_RANK} pathways into one group")
         if n_seg > MAX_SEG:
@@ -677,7 +679,7 @@ class GatherKernel:
         if n_tiles > 16384:
             raise NotImplementedError("more than 16384 target tiles")
         self.block = GATHER_BLOCK
-        self.events_per_thread = GATHER_EVENTS_PER_THREAD
+        self.events_per_thread = int(events_per_thread or GATHER_EVENTS_PER_THREAD)
         self.window = self.block * self.events_per_thread
         if not 1 <= pairs <= self.block:
             raise ValueError(f"pairs per unit must be in [1, {self.block}]")
@@ -761,7 +763,7 @@ class GatherKernel:
             else:
                 lines += translate_statement(st, types, declared, self.float_t)
         lines = _rename(lines, names)
-        k = [source_header(GT), ps.c_decl(), self.rec_decl]
+        k = [source_header(GT), DECL_MARK, ps.c_decl(), self.rec_decl]
         k.append(f'extern "C" __global__ void __launch_bounds__({GT}) {self.kernel_name}(const {ps.name} p) {{')
         k += [
             "    const long long step = p.clock[0] + p.step_off;",
@@ -1129,3 +1131,120 @@ class SummedKernel:
 
 __all__ = ["CodeSpec", "NeuronKernel", "GatherKernel", "PostPathKernel", "SummedKernel", "scalar_values",
            "identifiers", "c_literal"]
+
+
+# ---------------------------------------------------------------------------
+_SHARED_DECL = re.compile(r"^(\s*)__shared__ (?:__align__\((\d+)\) )?"
+                          r"((?:unsigned )?(?:long long|int|short|char|double|float|bool)) ([^;]+);(.*)$")
+_CT_BYTES = {"long long": 8, "unsigned long long": 8, "double": 8, "int": 4, "unsigned int": 4, "float": 4,
+             "short": 2, "unsigned short": 2, "char": 1, "unsigned char": 1, "bool": 1}
+
+
+def device_body(source, kernel_name, fname):
+    """A generated kernel as a __device__ function for a fused launch.
+
+    Returns (declarations, function, lds_bytes): the source's argument struct and
+    record declarations, and `fname(const S& p, long long hb_step_off, int hb_blk,
+    int hb_nblk, unsigned char* hb_lds)`, in which the workgroup index and count
+    are hb_blk and hb_nblk, the step offset is hb_step_off and every __shared__
+    variable is a reference into hb_lds (lds_bytes from its start): the bodies of
+    one fused kernel then share one LDS allocation instead of adding up."""
+    lines = source.split("\n")
+    start = next(i for i, x in enumerate(lines) if x.startswith('extern "C" __global__') and f" {kernel_name}(" in x)
+    m = re.search(r"\(const (\w+) p\) \{$", lines[start])
+    if m is None:
+        raise ValueError(f"unexpected signature of {kernel_name}")
+    # declarations: after the header (which ends with the prelude), before the kernel
+    head = "\n".join(lines[:start])
+    decls = head.split(DECL_MARK, 1)[1] if DECL_MARK in head else None
+    if decls is None:
+        raise ValueError(f"{kernel_name}: declarations are not marked")
+    out = [f"__device__ __forceinline__ void {fname}(const {m.group(1)}& p, long long hb_step_off, int hb_blk, "
+           "int hb_nblk, unsigned char* hb_lds) {"]
+    off = 0
+    for x in lines[start + 1:]:
+        d = _SHARED_DECL.match(x)
+        if d:
+            ind, al, ct, names, rest = d.groups()
+            nb = _CT_BYTES[ct]
+            refs = []
+            for nm in (n.strip() for n in names.split(",")):
+                a = re.fullmatch(r"(\w+)\[(\d+)\]", nm)
+                align = int(al) if al else nb
+                off = (off + align - 1) // align * align
+                if a:
+                    refs.append(f"{ct} (&{a.group(1)})[{a.group(2)}] = *reinterpret_cast<{ct} (*)[{a.group(2)}]>"
+                                f"(hb_lds + {off});")
+                    off += nb * int(a.group(2))
+                else:
+                    refs.append(f"{ct}& {nm} = *reinterpret_cast<{ct}*>(hb_lds + {off});")
+                    off += nb
+            out.append(ind + " ".join(refs) + rest)
+            continue
+        if "__shared__" in x:
+            raise ValueError(f"{kernel_name}: unrecognised LDS declaration: {x.strip()}")
+        x = x.replace("blockIdx.x", "hb_blk").replace("gridDim.x", "hb_nblk").replace("p.step_off", "hb_step_off")
+        out.append(x)
+    return decls, "\n".join(out), (off + 15) // 16 * 16
+
+
+class FusedStepKernel:
+    """One launch per step for a group and the gathers of its incoming pathways.
+
+    Valid when every incoming pathway's delays are at least one step: the gathers
+    of step s then read spikes of earlier steps only and write staging buffer
+    s % 3, which this step's fold does not read, so they are independent of the
+    group update in the same launch (the same independence that lets them run on
+    other streams).  Workgroups [0, sum(gather_blocks)) run the gathers, each
+    pathway on its own range of workgroups; the rest run the tiles of the group.
+    The bodies are the separate kernels' own code (device_body), sharing one LDS
+    allocation; the step offset is the fused argument struct's first field."""
+
+    def __init__(self, step, gathers, gather_blocks, n_tiles, tiles_first=False):
+        if step.flush or any(g.block != step.T for g in gathers):
+            raise ValueError("a fused launch needs a step kernel and gathers of one workgroup size")
+        if int(n_tiles) != -(-step.N // (step.T * step.npt)) or len(gather_blocks) != len(gathers):
+            raise ValueError("the fused launch's tiles or gather workgroups do not match its kernels")
+        self.step, self.gathers = step, list(gathers)
+        self.gather_blocks = [int(b) for b in gather_blocks]
+        self.n_tiles = int(n_tiles)
+        self.tiles_first = bool(tiles_first)
+        self.block = step.T
+        self.kernel_name = f"fused_{step.name}"
+        parts = [(step, f"hbb_step_{step.name}", "s")] + [
+            (g, f"hbb_gather_{g.name}", f"g{i}") for i, g in enumerate(self.gathers)]
+        decls, funcs, lds = [], [], 0
+        for K, fname, _ in parts:
+            d, f, nb = device_body(K.source, K.kernel_name, fname)
+            decls.append(d)
+            funcs.append(f)
+            lds = max(lds, nb)
+        self.lds_bytes = lds
+        if lds > LDS_BYTES:
+            raise ValueError("the fused kernel's LDS exceeds a CU's")
+        self.struct_name = f"hbf_{step.name}"
+        fields = ["    long long step_off;"] + [f"    {K.struct.name} {m};" for K, _, m in parts]
+        occ = f" __attribute__((amdgpu_waves_per_eu({step.waves_per_eu})))" if step.waves_per_eu else ""
+        k = [source_header(step.T)] + decls + funcs
+        k.append(f"struct {self.struct_name} {{\n" + "\n".join(fields) + "\n};")
+        k.append(f'extern "C" __global__ void __launch_bounds__({step.T}){occ} {self.kernel_name}'
+                 f"(const {self.struct_name} a) {{")
+        k.append(f"    __shared__ __align__(16) unsigned char hb_lds[{lds}];")
+        k.append("    int b = (int)blockIdx.x;")
+        tiles = f"    if (b < {self.n_tiles}) {{ hbb_step_{step.name}(a.s, a.step_off, b, {self.n_tiles}, hb_lds); return; }}"
+        if self.tiles_first:
+            k += [tiles, f"    b -= {self.n_tiles};"]
+        for i, (g, nb) in enumerate(zip(self.gathers, self.gather_blocks)):
+            k.append(f"    if (b < {nb}) {{ hbb_gather_{g.name}(a.g{i}, a.step_off, b, {nb}, hb_lds); return; }}")
+            k.append(f"    b -= {nb};")
+        if not self.tiles_first:
+            k.append(tiles)
+        k.append("}")
+        self.source = "\n".join(k) + "\n"
+        self.grid = self.n_tiles + sum(self.gather_blocks)
+
+    def ctypes_struct(self, step_cls, gather_clss):
+        import ctypes
+
+        fields = [("step_off", ctypes.c_int64), ("s", step_cls)] + [(f"g{i}", c) for i, c in enumerate(gather_clss)]
+        return type(self.struct_name, (ctypes.Structure,), {"_fields_": fields})

@@ -21,7 +21,9 @@ from . import _lib
 from . import kernels as _kernels
 from .codegen import ctype_of
 from .kernels import (
+    LDS_BYTES,
     CodeSpec,
+    FusedStepKernel,
     GatherKernel,
     NeuronKernel,
     PostPathKernel,
@@ -102,6 +104,16 @@ class Device:
         r, sc, lds = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
         _lib.check(L.hb_function_info(fn, ctypes.byref(r), ctypes.byref(sc), ctypes.byref(lds)), "hb_function_info")
         return r.value, sc.value, lds.value
+
+    def spills(self, fn):
+        """Whether a generated kernel spills vector registers to memory: by the code object's
+        VGPR spill count, or by its private segment when that is unknown (scalar registers
+        spill into vector register lanes)."""
+        v, s = ctypes.c_int(), ctypes.c_int()
+        _lib.check(L.hb_function_spills(fn, ctypes.byref(v), ctypes.byref(s)), "hb_function_spills")
+        if v.value < 0:
+            return self.function_info(fn)[1] > 0
+        return v.value > 0
 
     def sync(self):
         _torch().cuda.synchronize()
@@ -648,15 +660,19 @@ class Network:
     """Runs the step loop on the device (Network.run, brian2/core/network.py:1047-1292)."""
 
     def __init__(self, *objs, dt=1e-4, bucket_cap=None, graph_steps=100, bucket_mem_frac=0.4, tile=None,
-                 gather_pairs=None, npt=None, overlap=True):
+                 gather_pairs=None, npt=None, overlap=True, fuse=True):
         """bucket_cap: events of a tile sorted in LDS at a time (default: what fits two
         workgroups per CU); bucket_mem_frac: share of free device memory the event
         staging may use; tile: threads per workgroup of the step kernels (default:
         1024 for groups of at least 65536 neurons, else 256); npt: neurons per thread
         (default 1); overlap: run the gathers concurrently with the group updates when
-        every delay is at least one step (default True); gather_pairs: (delay, spike)
-        pairs per gather unit (default: about one register window of events).  These
-        shape the launches only: results do not depend on them."""
+        every delay is at least one step (default True); fuse: with overlap, run a
+        group of 1024-thread tiles and the gathers into it as one launch per step
+        (default True; a dict sets the fused launch's shape: events_per_thread and
+        gather_blocks of its gathers, tiles_first, allow_spills); gather_pairs: (delay,
+        spike) pairs per gather unit (default:
+        about one register window of events).  These shape the launches only: results
+        do not depend on them."""
         self.objs = []
         self.dt = float(dt)
         self.bucket_cap = None if bucket_cap is None else int(bucket_cap)
@@ -666,6 +682,16 @@ class Network:
         self.npt = npt
         self.gather_pairs = gather_pairs
         self.overlap = bool(overlap)
+        self.fuse = bool(fuse)
+        # fused launches: gather events per thread and workgroups per pathway (None: by the
+        # build), tiles before gathers in the grid, a two-per-CU build kept despite spills
+        self.fuse_cfg = {"events_per_thread": None, "gather_blocks": None, "tiles_first": False,
+                         "allow_spills": False}
+        if isinstance(fuse, dict):
+            unknown = set(fuse) - set(self.fuse_cfg)
+            if unknown:
+                raise TypeError(f"unknown fused-launch settings {sorted(unknown)}")
+            self.fuse_cfg.update(fuse)
         self.t_step = 0
         self._built = False
         for o in objs:
@@ -738,6 +764,17 @@ class Network:
         self._plan = ctypes.c_void_p()
         _lib.check(L.hb_plan_create(D.ctx, ptr(self.clock), ctypes.byref(self._plan)), "hb_plan_create")
         self._launches = []
+        # with every delay at least one step, the gathers of step s read only spikes of earlier
+        # steps: they run concurrently with the group updates of step s and with each other
+        # (three side lanes, taken in turn), or inside the target group's launch
+        self.concurrent_gathers = (bool(paths) and all(s.csr["dmin"] >= 1 for s in paths) and self.overlap
+                                   and not any(s.plastic for s in paths))
+        for g in groups:
+            g.fused = None
+            g.fuse_ok = (self.fuse and self.concurrent_gathers and isinstance(g, NeuronGroup)
+                         and bool(g.incoming) and g.block == _kernels.GATHER_BLOCK)
+        for s in paths:
+            s.fused_into = s.target if s.target.fuse_ok else None
         # summed variables first: their updaters run in the 'groups' slot before the target's
         # state update (synapses.py:146-160, order target.order - 1)
         for s in self.summed_syn:
@@ -756,18 +793,14 @@ class Network:
             else:
                 self._build_spikegen(g)
         for s in paths:
-            self._build_gather(s)
+            if s.fused_into is None:
+                self._build_gather(s)
         # on_post pathways after every on_pre pathway of the step (synapses.py:205-216)
         for s in paths:
             if s.post_spec is not None:
                 self._build_post(s)
-        # with every delay at least one step, the gathers of step s read only spikes of earlier
-        # steps: they run concurrently with the group updates of step s and with each other
-        # (three side lanes, taken in turn)
-        self.concurrent_gathers = (bool(paths) and all(s.csr["dmin"] >= 1 for s in paths) and self.overlap
-                                   and not any(s.plastic for s in paths))
         if self.concurrent_gathers:
-            for i, s in enumerate(paths):
+            for i, s in enumerate([s for s in paths if s.fused_into is None]):
                 _lib.check(L.hb_plan_set_lane(self._plan, s.launch_index, 1 + i % 3), "hb_plan_set_lane")
         self._built = True
 
@@ -796,13 +829,7 @@ class Network:
         window = _kernels.GATHER_BLOCK * _kernels.GATHER_EVENTS_PER_THREAD
         total = sum(int(s.csr["n_syn"]) for s in incoming)
         for s in incoming:
-            nsrc = s.src_range[1] - s.src_range[0]
-            pairs_max = nsrc * s.csr["n_seg"]
-            # pairs per unit: about one window of events per unit on average
-            avg = s.csr["n_syn"] / max(1, pairs_max)
-            pc = int(min(_kernels.GATHER_BLOCK, max(32, 0.85 * window / max(avg, 1e-9))))
-            s.pairs = int(self.gather_pairs or pc)
-            s.units_max = -(-pairs_max // s.pairs)
+            self._gather_units(s, window)
         indeg_n = torch.zeros(g.N, dtype=torch.int64, device="cuda")
         for s in incoming:
             post = s.csr["post"][: s.csr["n_syn"]].long()
@@ -826,6 +853,16 @@ class Network:
         g.tile_cnt = zeros(3 * g.n_tiles, "int")
         g.stg_rec = torch.empty(3 * g.stg_cap * rec_words(g.val_t), dtype=torch.int32, device="cuda")
         g.stg_ord = torch.empty(max(1, g.n_tiles * g.tile_cap), dtype=torch.int32, device="cuda")
+
+    def _gather_units(self, s, window):
+        """(delay, spike) pairs per gather unit: about one window of events per unit on
+        average (s.pairs), and the most units one step can have (s.units_max)."""
+        nsrc = s.src_range[1] - s.src_range[0]
+        pairs_max = nsrc * s.csr["n_seg"]
+        avg = s.csr["n_syn"] / max(1, pairs_max)
+        pc = int(min(_kernels.GATHER_BLOCK, max(32, 0.85 * window / max(avg, 1e-9))))
+        s.pairs = int(self.gather_pairs or pc)
+        s.units_max = -(-pairs_max // s.pairs)
 
     def _value_type(self, s):
         from .codegen import Renderer
@@ -903,12 +940,15 @@ class Network:
         # two workgroups of 1024 threads per CU need at most 64 VGPRs: keep that build only
         # if the compiler fits it without spilling
         g.waves = 8 if getattr(g, "two_per_cu", False) else 0
-        K = step_kernel(g.waves)
-        g.fn = D.function(K.source, K.kernel_name)
-        if g.waves and D.function_info(g.fn)[1] > 0:
-            g.waves = 0
-            K = step_kernel(0)
+        if g.fuse_ok:
+            K = self._build_fused(g, step_kernel)
+        else:
+            K = step_kernel(g.waves)
             g.fn = D.function(K.source, K.kernel_name)
+            if g.waves and D.spills(g.fn):
+                g.waves = 0
+                K = step_kernel(0)
+                g.fn = D.function(K.source, K.kernel_name)
         g.kernel = K
         g.struct_cls = K.struct.ctypes_struct()
         if g.smon:
@@ -918,8 +958,14 @@ class Network:
             m.rec_idx = dev(m.sorted_idx, "int")
             m.rec_buf = None
         g.launch_index = len(self._launches)
-        args = self._group_args(g)
-        self._add_launch(g.fn, (g.N + g.tile - 1) // g.tile, g.block, args, g.struct_cls)
+        if g.fused is not None:
+            F = g.fused
+            g.fused_cls = F.ctypes_struct(g.struct_cls, [s.struct_cls for s in g.incoming])
+            for s in g.incoming:
+                s.launch_index = g.launch_index
+            self._add_launch(g.fn, F.grid, F.block, self._launch_args(g), g.fused_cls)
+        else:
+            self._add_launch(g.fn, (g.N + g.tile - 1) // g.tile, g.block, self._group_args(g), g.struct_cls)
         end_mon = g.stmon is not None and g.stmon.when == "end"
         if g.incoming or mode == "deferred" or end_mon:
             F = NeuronKernel(g.name, g.N, g.vars, g.float_t, g.su, g.th, g.rs, mode, scalars, fold,
@@ -931,6 +977,51 @@ class Network:
             g.flush_struct_cls = F.struct.ctypes_struct()
         else:
             g.flush_fn = None
+
+    def _build_fused(self, g, step_kernel):
+        """The step kernel of g and the gathers of its incoming pathways as one kernel
+        (kernels.FusedStepKernel).  Two workgroups per CU when the whole kernel fits
+        in 64 VGPRs without spilling, with gather windows of 4, 3 or 2 events per
+        thread (the largest that fits; their LDS then fits beside the fold's share),
+        else one, with the gathers' usual windows."""
+        D = self.device
+        cfg = self.fuse_cfg
+        ept = cfg["events_per_thread"]
+        tries = [(g.waves, u) for u in ((ept,) if ept else (4, 3, 2))] if g.waves else []
+        tries.append((0, ept or _kernels.GATHER_EVENTS_PER_THREAD))
+        for k, (waves, ept) in enumerate(tries):
+            K = step_kernel(waves)
+            for s in g.incoming:
+                self._gather_units(s, _kernels.GATHER_BLOCK * ept)
+                self._gather_setup(s, ept)
+            # a quarter of the workgroup slots per pathway (measured on the 1M-neuron network:
+            # fewer starve a pathway of workgroups, more delay the tiles)
+            wgs = 2 if waves else 1
+            blocks = cfg["gather_blocks"] or [max(1, self.device.n_cu * wgs // 4)] * len(g.incoming)
+            blocks = [int(max(1, min(b, s.units_max))) for b, s in zip(blocks, g.incoming)]
+            F = FusedStepKernel(K, [s.kernel for s in g.incoming], blocks, g.n_tiles, cfg["tiles_first"])
+            fn = D.function(F.source, F.kernel_name)
+            spills = D.spills(fn) and not cfg["allow_spills"]
+            if waves and k + 1 < len(tries) and (spills or F.lds_bytes > LDS_BYTES // 2):
+                continue
+            g.waves, g.fn, g.fused = waves, fn, F
+            for s, b in zip(g.incoming, blocks):
+                s.fn, s.grid = None, b
+                s.events_dev = _torch().zeros(b, dtype=_torch().int64, device="cuda")
+                s.args.events = s.events_dev.data_ptr()
+            return K
+        raise AssertionError("unreachable")
+
+    def _launch_args(self, g):
+        """Argument struct of g's launch: its step kernel's, or the fused launch's."""
+        if g.fused is None:
+            return self._group_args(g)
+        a = g.fused_cls()
+        a.step_off = 0
+        a.s = self._group_args(g)
+        for i, s in enumerate(g.incoming):
+            setattr(a, f"g{i}", s.args)
+        return a
 
     def _alloc_spike_monitor(self, g, m):
         m.cap = max(1 << 20, 2 * g.N * self.graph_steps)
@@ -1044,7 +1135,16 @@ class Network:
             self.status.data_ptr())
 
     def _build_gather(self, s):
-        D = self.device
+        self._gather_setup(s)
+        s.fn = self.device.function(s.kernel.source, s.kernel.kernel_name)
+        s.events_dev = _torch().zeros(s.grid, dtype=_torch().int64, device="cuda")
+        s.args.events = s.events_dev.data_ptr()
+        s.launch_index = len(self._launches)
+        self._add_launch(s.fn, s.grid, s.kernel.block, s.args, s.struct_cls)
+
+    def _gather_setup(self, s, events_per_thread=None):
+        """The gather kernel of pathway s (not compiled), its synapse records and its
+        argument struct (s.kernel, s.args, s.struct_cls, s.grid)."""
         g = s.target
         rank = g.incoming.index(s)
         host = {"dt": self.dt}
@@ -1053,9 +1153,9 @@ class Network:
         types_syn = s.syn_ctypes()
         K = GatherKernel(s.name, rank, s.csr["n_seg"], s.csr["dmin"], s.spec.vector, types_syn, sc, g.val_t,
                          g.float_t, s.target_var, g.n_tiles, g.tile, count_synapses=s.count_events,
-                         pairs=s.pairs, target_index=s.target_index, plastic=s.plastic)
+                         pairs=s.pairs, target_index=s.target_index, plastic=s.plastic,
+                         events_per_thread=events_per_thread)
         s.kernel = K
-        s.fn = D.function(K.source, K.kernel_name)
         s.csr["rec"] = self._synapse_records(s, K.rec_arrays)
         if s.csr["rec"].numel() != max(1, s.csr["n_syn"]) * K.rec_words and s.csr["n_syn"] > 0:
             raise RuntimeError("synapse records do not match the gather's layout")
@@ -1075,18 +1175,15 @@ class Network:
                 v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
         # enough workgroups to fill every CU (units are taken in turn)
         grid = int(min(self.device.n_cu * K.wgs_per_cu, max(1, s.units_max)))
-        s.events_dev = _torch().zeros(grid, dtype=_torch().int64, device="cuda")
-        v["events"] = s.events_dev.data_ptr()
-        if s.count_events:
+        if s.count_events and getattr(s, "syn_count_dev", None) is None:
             s.syn_count_dev = zeros(s.csr["n_syn"], "int")
+        if s.count_events:
             v["syn_count"] = s.syn_count_dev.data_ptr()
         a = cls()
         for name, _ in cls._fields_:
             if name in v:
                 setattr(a, name, v[name])
-        s.launch_index = len(self._launches)
         s.args, s.grid = a, grid
-        self._add_launch(s.fn, grid, K.block, a, cls)
 
     def _build_summed(self, s):
         g = s.target
@@ -1275,13 +1372,13 @@ class Network:
                 m.row0 = self.t_step
                 m.rec_buf = torch.zeros(max(n_steps, 1) * len(m.variables) * len(m.sorted_idx), dtype=torch.float64,
                                         device="cuda")
-                self._set_launch_args(g.launch_index, self._group_args(g))
+                self._set_launch_args(g.launch_index, self._launch_args(g))
             # population rate monitors: spike counts of this run's steps
             m = getattr(g, "rmon", None)
             if m is not None and not isinstance(g, SpikeGeneratorGroup):
                 m.rows, m.row0 = n_steps, self.t_step
                 m.buf = zeros(max(n_steps, 1), "int")
-                self._set_launch_args(g.launch_index, self._group_args(g))
+                self._set_launch_args(g.launch_index, self._launch_args(g))
         done = 0
         while done < n_steps:
             chunk = n_steps - done
@@ -1350,8 +1447,12 @@ class Network:
             raise _lib.HipBrianError(self._status_message(st))
         out = []
         for obj in list(self.groups) + list(self.paths):
-            kind = "gather" if isinstance(obj, Synapses) else "step"
-            out.append((kind, obj.name, float(ms[obj.launch_index])))
+            if isinstance(obj, Synapses):
+                if obj.fused_into is None:
+                    out.append(("gather", obj.name, float(ms[obj.launch_index])))
+            else:
+                kind = "fused" if getattr(obj, "fused", None) is not None else "step"
+                out.append((kind, obj.name, float(ms[obj.launch_index])))
         return out
 
     @staticmethod
@@ -1396,7 +1497,7 @@ class Network:
             ni[:n] = m.mon_i[:n]
             ns[:n] = m.mon_step[:n]
             m.mon_i, m.mon_step, m.cap = ni, ns, newcap
-            self._set_launch_args(g.launch_index, self._group_args(g))
+            self._set_launch_args(g.launch_index, self._launch_args(g))
             free = m.cap - n
         return max(1, free // max(g.N, 1))
 

@@ -71,6 +71,14 @@ int hb_module_function(hb_module* mod, const char* name, void** fn);
  * runtime compiles once per code object); the runtime uses it to pick between a
  * two-workgroups-per-CU build of a step kernel and a one-workgroup build. */
 int hb_function_info(void* fn, int* vgprs, int64_t* scratch_bytes, int64_t* lds_bytes);
+/* Registers spilled to memory by a generated kernel's compilation (VGPRs and SGPRs,
+ * the code object's .vgpr_spill_count / .sgpr_spill_count; -1 when its metadata does not
+ * say).  A private segment can be reserved with nothing spilled, so this, not the
+ * scratch size, decides whether a build for two workgroups per CU is kept.  No
+ * counterpart in Brian. */
+int hb_function_spills(void* fn, int* vgpr_spills, int* sgpr_spills);
+/* The same counts read from a code object in memory (largest over its kernels). */
+int hb_code_object_spills(const void* code, size_t nbytes, int* vgpr_spills, int* sgpr_spills);
 /* Launches a module function with a packed argument buffer (the kernel takes
  * one struct by value whose bytes are `args`). */
 int hb_launch(void* fn, unsigned grid, unsigned block, unsigned shmem, void* stream, const void* args,

@@ -189,3 +189,36 @@ def test_step_kernel_two_workgroups_per_cu_build():
                      waves_per_eu=8)
     assert "amdgpu_waves_per_eu(8)" in K.source
     _compile(K.source)
+
+
+def test_fused_step_and_gathers_compile_for_gfx950():
+    """A group's step kernel and the gathers into it as one kernel: the bodies share
+    one LDS allocation (references into it, none of their own), use the fused
+    launch's workgroup ranges and step offset, and the two-per-CU build keeps the
+    LDS within half a CU."""
+    from brian2_amd.kernels import LDS_BYTES, FusedStepKernel, fold_capacity
+
+    m = Model.load("cuba_synw")
+    N = 5000
+    G = NeuronGroup(N, m, float_t="double")
+    cap = fold_capacity(1024, "float", 1, wgs_per_cu=2)
+    K = NeuronKernel("g", N, G.vars, "double", G.su, G.th, G.rs, "fused", _scalars(G),
+                     [(0, "ge", "+="), (1, "gi", "+=")], cap, "float", spikemon=(0, N), tile=1024, waves_per_eu=8)
+    nt = -(-N // 1024)
+    gs = [GatherKernel(f"p{r}", r, 50, 1, m.pathways[pw].vector, {"w": "float"},
+                       scalar_values(m.pathways[pw], {"dt": 1e-4}, "double"), "float", "double", tv, nt, 1024,
+                       events_per_thread=2)
+          for r, (pw, tv) in enumerate((("exc", "ge"), ("inh", "gi")))]
+    for tiles_first in (False, True):
+        F = FusedStepKernel(K, gs, [7, 3], nt, tiles_first=tiles_first)
+        assert F.grid == nt + 10 and F.lds_bytes <= LDS_BYTES // 2
+        body = F.source.split("struct hbf_g", 1)[0]
+        funcs = body[body.index("__device__ __forceinline__ void hbb_step_g"):]
+        assert "__shared__" not in funcs and "blockIdx" not in funcs and "gridDim" not in funcs
+        assert "p.step_off" not in funcs
+        _no_float_atomics(F.source)
+        _compile(F.source)
+    with pytest.raises(ValueError):
+        FusedStepKernel(K, [GatherKernel("q", 0, 1, 1, m.pathways["exc"].vector, {"w": "float"},
+                                         scalar_values(m.pathways["exc"], {"dt": 1e-4}, "double"), "float",
+                                         "double", "ge", nt, 1024)], [1], nt + 1)

@@ -55,11 +55,20 @@ def test_burst_trace_bitexact(case, shape):
     assert np.array_equal(got, exp)
 
 
-@pytest.mark.parametrize("shape", [{}, {"tile": 64, "bucket_cap": 128, "gather_pairs": 32}, {"overlap": False}],
-                         ids=["default", "tile64-cap128-pairs32", "no-overlap"])
+@pytest.mark.parametrize("shape", [
+    {},
+    {"tile": 64, "bucket_cap": 128, "gather_pairs": 32},
+    {"overlap": False},
+    {"tile": 1024},
+    {"tile": 1024, "fuse": False},
+    {"tile": 1024, "fuse": {"allow_spills": True, "events_per_thread": 2}},
+    {"tile": 1024, "bucket_cap": 128, "gather_pairs": 32, "fuse": {"tiles_first": True, "gather_blocks": [3]}},
+], ids=["default", "tile64-cap128-pairs32", "no-overlap", "tile1024-fused", "tile1024-lanes",
+        "tile1024-fused-2percu", "tile1024-fused-tilesfirst-cap128"])
 def test_burst_trace_bitexact_with_concurrent_gathers(case, shape):
     """Delays 1-21 steps: the gathers of a step run concurrently with the group
-    update of that step (triple-buffered event staging)."""
+    update of that step (triple-buffered event staging), on other streams or, for
+    tiles of 1024 threads, in the group's launch."""
     pre, post, w, d, _ = case
     d1 = d + 1
     exp = expected_trace(pre, post, w, d1)

@@ -61,3 +61,47 @@ def test_one_hip_runtime_in_the_process():
     with open("/proc/self/maps") as f:
         libs = {ln.split()[-1] for ln in f if "libamdhip64" in ln}
     assert len(libs) == 1, libs
+
+
+def test_code_object_spill_counts():
+    """hb_code_object_spills reads the spill counts a gfx950 code object records: none for
+    a small kernel, some for one held to 32 VGPRs with many live values."""
+    import subprocess
+    import tempfile
+
+    from brian2_amd import _lib
+
+    small = r"""
+#include <hip/hip_runtime.h>
+extern "C" __global__ void __launch_bounds__(64) small(float* x) { x[threadIdx.x] += 1.0f; }
+"""
+    big = r"""
+#include <hip/hip_runtime.h>
+extern "C" __global__ void __launch_bounds__(1024) __attribute__((amdgpu_num_vgpr(32))) big(float* x) {
+    float a[96];
+    #pragma unroll
+    for (int i = 0; i < 96; i++) a[i] = x[threadIdx.x + 1024 * i];
+    float s = 0.f;
+    #pragma unroll
+    for (int i = 0; i < 96; i++) s += a[i] * a[95 - i] + a[(i * 7) % 96];
+    #pragma unroll
+    for (int i = 0; i < 96; i++) x[threadIdx.x + 1024 * i] = a[i] * s;
+}
+"""
+    counts = {}
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR")) as d:
+        for name, src in (("small", small), ("big", big)):
+            p = os.path.join(d, f"{name}.hip")
+            with open(p, "w") as f:
+                f.write(src)
+            co = os.path.join(d, f"{name}.co")
+            r = subprocess.run(["hipcc", "--offload-arch=gfx950", "-O3", "--genco", p, "-o", co],
+                               capture_output=True, text=True)
+            assert r.returncode == 0, r.stderr[-2000:]
+            with open(co, "rb") as f:
+                data = f.read()
+            v, s = ctypes.c_int(), ctypes.c_int()
+            assert _lib.lib.hb_code_object_spills(data, len(data), ctypes.byref(v), ctypes.byref(s)) == 0
+            counts[name] = (v.value, s.value)
+    assert counts["small"] == (0, 0)
+    assert counts["big"][0] > 0
