@@ -23,7 +23,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 Third leg: the differentiable LIF network (256 trials of 200 ms): forward and
 exact adjoint backward, trials x simulated seconds per second.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-sweep] [--no-adjoint] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-fuse] [--no-sweep] [--no-adjoint] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -189,6 +189,8 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=100)
     ap.add_argument("--float", dest="float_t", default="double", choices=["double", "float"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="the group step and each pathway's gather as separate launches (gathers on side streams)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -208,7 +210,8 @@ def main():
     if args.copy_calibration:
         copy_calibration()
     t0 = time.perf_counter()
-    c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=args.graph_steps)
+    c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=args.graph_steps,
+                                network_options={"fuse": not args.no_fuse})
     torch.cuda.synchronize()
     log(f"built network: {c['n_syn']} synapses in {time.perf_counter() - t0:.1f} s")
     net, mon = c["net"], c["mon"]
