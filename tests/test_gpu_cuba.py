@@ -74,14 +74,16 @@ def test_cuba_fp32_mode_error_reported():
     assert abs(n32 - n64) <= 0.05 * n64
 
 
-def test_store_restore_replays_bit_identically():
+@pytest.mark.parametrize("shape", [{}, {"tile": 1024}], ids=["default", "tile1024-fused"])
+def test_store_restore_replays_bit_identically(shape):
     """Network.store/restore (brian2/core/network.py:600-740): a restored network
     repeats the same spikes, state and event counts; profile() (one launch at a
     time) gives the same results as graph replay."""
     from brian2_amd import synth
 
     z, meta = O.load_golden("cuba_delay")
-    c = synth.cuba_synw_network(N=meta["N"], K=meta["K"], seed=meta["seed"], ginh=meta["ginh"])
+    c = synth.cuba_synw_network(N=meta["N"], K=meta["K"], seed=meta["seed"], ginh=meta["ginh"],
+                                network_options=shape)
     net, mon = c["net"], c["mon"]
     net.run(300)
     net.store()
@@ -99,7 +101,8 @@ def test_store_restore_replays_bit_identically():
     assert all(np.array_equal(c["G"].get(v), x) for v, x in zip(("v", "ge", "gi"), a_state))
     assert net.n_events - ev0 == a_ev
     net.restore()
-    net.profile(400)
+    prof = net.profile(400)
+    assert [k for k, _, _ in prof] == (["fused"] if shape else ["step", "gather", "gather"])
     net.run(0)  # folds the events of the last profiled step
     assert np.array_equal(mon.i[n0:], a_i) and np.array_equal(mon.step[n0:], a_s)
     assert all(np.array_equal(c["G"].get(v), x) for v, x in zip(("v", "ge", "gi"), a_state))

@@ -139,3 +139,46 @@ def test_rate_monitor_of_a_spike_generator():
         if 1 <= i < 8:
             expect[t] += 1
     assert np.array_equal(rm.count, expect) and np.array_equal(rm.step, np.arange(16))
+
+
+@pytest.mark.parametrize("shape", [{}, {"tile": 1024}, {"tile": 1024, "fuse": False}],
+                         ids=["default", "tile1024-fused", "tile1024-separate"])
+def test_poisson_two_groups_three_pathways_like_the_oracle(shape):
+    """PoissonGroup -> two pathways (delays 1-4 and 2-7 steps) into one group and a third
+    (1-3) into another: with tiles of 1024 each group and the gathers into it run as one
+    launch per step (two of them, each reading the source's spike ring); final state
+    bit-exact against the oracle's sequential fold in pathway order."""
+    from brian2_amd import Network, NeuronGroup, PoissonGroup, StateMonitor, Synapses
+
+    n_src, n1, n2, steps, seed = 3000, 300, 200, 500, 13
+    rates = _rates(n_src, seed=31)
+    rng = np.random.default_rng(8)
+    P = PoissonGroup(n_src, rates, seed=seed)
+    T1 = NeuronGroup(n1, variables={"v": "float64"}, name="t1")
+    T2 = NeuronGroup(n2, variables={"v": "float64"}, name="t2")
+    src = O.OracleGroup(O.HBR_POISSON, n_src, np.zeros(n_src), [], refractory=False, poisson=(seed, rates * DT))
+    o1 = O.OracleGroup(O.HBR_PASSIVE, n1, np.zeros(n1), [], refractory=False)
+    o2 = O.OracleGroup(O.HBR_PASSIVE, n2, np.zeros(n2), [], refractory=False)
+    objs, opaths = [P, T1, T2], []
+    for name, tgt, otgt, n_syn, d0, d1 in (("a", T1, o1, 30000, 1, 4), ("b", T1, o1, 20000, 2, 7),
+                                           ("c", T2, o2, 25000, 1, 3)):
+        pre = rng.integers(0, n_src, n_syn).astype(np.int32)
+        post = rng.integers(0, tgt.N, n_syn).astype(np.int32)
+        w = rng.normal(0.0, 1.0, n_syn)
+        d = rng.integers(d0, d1 + 1, n_syn).astype(np.float64) * DT
+        S = Synapses(P, tgt, onpre_spec("w", ("w",)), name=name)
+        S.connect(pre, post)
+        S.set("w", w)
+        S.set_delay(d)
+        objs.append(S)
+        opaths.append(O.OraclePathway(src, otgt, pre, post, 0, w=w, delay=d))
+    st1 = StateMonitor(T1, ["v"], record=np.arange(n1), when="end")
+    st2 = StateMonitor(T2, ["v"], record=np.arange(n2), when="end")
+    print(f"poisson -> 2 groups, 3 pathways, launch shape {shape}", flush=True)
+    net = Network(*objs, st1, st2, dt=DT, **shape)
+    net.run(steps)
+    if shape.get("tile") == 1024 and shape.get("fuse", True):
+        assert T1.fused is not None and T2.fused is not None
+    O.run([src, o1, o2], opaths, DT, steps, mon_group=None)
+    assert np.array_equal(st1.values()[0][:, -1], o1.state)
+    assert np.array_equal(st2.values()[0][:, -1], o2.state)
