@@ -682,7 +682,7 @@ class Network:
         self.npt = npt
         self.gather_pairs = gather_pairs
         self.overlap = bool(overlap)
-        self.fuse = bool(fuse)
+        self.fuse = isinstance(fuse, dict) or bool(fuse)
         # fused launches: gather events per thread and workgroups per pathway (None: by the
         # build), tiles before gathers in the grid, a two-per-CU build kept despite spills
         self.fuse_cfg = {"events_per_thread": None, "gather_blocks": None, "tiles_first": False,
@@ -692,6 +692,9 @@ class Network:
             if unknown:
                 raise TypeError(f"unknown fused-launch settings {sorted(unknown)}")
             self.fuse_cfg.update(fuse)
+        gb = self.fuse_cfg["gather_blocks"]
+        if gb is not None and (not isinstance(gb, (list, tuple)) or any(int(b) < 1 for b in gb)):
+            raise ValueError("gather_blocks: a list of positive workgroup counts, one per pathway into the group")
         self.t_step = 0
         self._built = False
         for o in objs:
@@ -998,6 +1001,9 @@ class Network:
             # fewer starve a pathway of workgroups, more delay the tiles)
             wgs = 2 if waves else 1
             blocks = cfg["gather_blocks"] or [max(1, self.device.n_cu * wgs // 4)] * len(g.incoming)
+            if len(blocks) != len(g.incoming):
+                raise ValueError(f"gather_blocks has {len(blocks)} entries for {len(g.incoming)} pathways into "
+                                 f"{g.name}")
             blocks = [int(max(1, min(b, s.units_max))) for b, s in zip(blocks, g.incoming)]
             F = FusedStepKernel(K, [s.kernel for s in g.incoming], blocks, g.n_tiles, cfg["tiles_first"])
             fn = D.function(F.source, F.kernel_name)

@@ -222,3 +222,17 @@ def test_fused_step_and_gathers_compile_for_gfx950():
         FusedStepKernel(K, [GatherKernel("q", 0, 1, 1, m.pathways["exc"].vector, {"w": "float"},
                                          scalar_values(m.pathways["exc"], {"dt": 1e-4}, "double"), "float",
                                          "double", "ge", nt, 1024)], [1], nt + 1)
+
+
+def test_network_fuse_options():
+    """Network(fuse=...) settings are checked on the host (no device needed)."""
+    from brian2_amd.runtime import Network
+
+    assert Network(fuse={}).fuse and not Network(fuse=False).fuse and Network().fuse
+    assert Network(fuse={"events_per_thread": 3, "gather_blocks": [128, 128]}).fuse_cfg["gather_blocks"] == [128, 128]
+    with pytest.raises(TypeError):
+        Network(fuse={"blocks": 3})
+    with pytest.raises(ValueError):
+        Network(fuse={"gather_blocks": 128})
+    with pytest.raises(ValueError):
+        Network(fuse={"gather_blocks": [0, 4]})
