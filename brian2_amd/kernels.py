@@ -369,7 +369,7 @@ class NeuronKernel:
     def __init__(self, name, N, arrays, float_t, stateupdate=None, threshold=None, reset=None,
                  reset_mode="none", scalars=None, fold=None, lds_cap=0, val_t="double",
                  spikemon=None, statemon=None, flush=False, uses_refractory=False, statemon_when="start",
-                 tile=256, npt=1, ratemon=None, waves_per_eu=0):
+                 tile=256, npt=1, ratemon=None, waves_per_eu=0, ring_parts=None):
         self.name = name
         self.N = N
         self.arrays = arrays          # {var: ctype} of the group's state arrays
@@ -389,6 +389,11 @@ class NeuronKernel:
         self.statemon_when = statemon_when
         self.flush = flush
         self.uses_refractory = uses_refractory
+        # starts of the parts of the spike ring (neuron ranges, ascending, the first 0): the
+        # spikes of part r go to slot * stride + start_r, counted in ring_cnt[slot * R + r]
+        self.ring_parts = tuple(ring_parts or (0,))
+        if self.ring_parts[0] != 0 or list(self.ring_parts) != sorted(set(self.ring_parts)) or self.ring_parts[-1] >= max(N, 1):
+            raise ValueError("ring parts must be ascending neuron indices starting at 0")
         self.T = tile                 # threads per workgroup
         self.npt = npt                # neurons per thread: a workgroup owns tile * npt neurons
         if npt < 1 or tile * npt > 65536:
@@ -476,6 +481,10 @@ class NeuronKernel:
         if not self.flush:
             # spike counts of the workgroup (cleared here, before the first barrier)
             k += ["    __shared__ int s_nsp, s_nspm, s_nrt;", "    if (threadIdx.x == 0) { s_nsp = 0; s_nspm = 0; s_nrt = 0; }"]
+            if len(self.ring_parts) > 1:
+                R = len(self.ring_parts)
+                k += [f"    __shared__ int s_npart[{R}], s_pbase[{R}];",
+                      f"    if (threadIdx.x < {R}) s_npart[threadIdx.x] = 0;"]
         fold = _fold_source(T, self.cap, self.val_t, self.flush, NPT) if self.fold else None
         if fold:
             k += fold[0]  # the tile's event count first: phase A's loads depend on it, not on the state
@@ -584,8 +593,26 @@ class NeuronKernel:
                 "            hb_nm += (hb_condq[q] && i >= p.mon_start && i < p.mon_stop) ? 1 : 0;",
                 "        }",
             ]
-        packed = self.spikemon and TN < 32768
-        if packed:
+        RPARTS = self.ring_parts
+        R = len(RPARTS)
+        bounds = list(RPARTS) + [self.N]
+        if R > 1:
+            k += ["        // spikes per part of the ring"]
+            for r in range(R):
+                k += [
+                    f"        int hb_np{r} = 0;",
+                    "        #pragma unroll",
+                    f"        for (int q = 0; q < {NPT}; q++) {{",
+                    f"            const int i = hb_i0 + q * {T};",
+                    f"            hb_np{r} += (hb_condq[q] && i >= {bounds[r]} && i < {bounds[r + 1]}) ? 1 : 0;",
+                    "        }",
+                    f"        int hb_op{r} = hb_block_alloc(hb_np{r}, &s_npart[{r}]);",
+                ]
+        packed = self.spikemon and TN < 32768 and R == 1
+        if R > 1:
+            if self.spikemon:
+                k.append("        const int hb_offm = hb_block_alloc(hb_nm, &s_nspm);")
+        elif packed:
             k += [
                 "        // both counts in one allocation (each total is at most the tile size, below 2^15)",
                 "        const int hb_po = hb_block_alloc(hb_n | (hb_nm << 16), &s_nsp);",
@@ -606,12 +633,14 @@ class NeuronKernel:
                 "        }",
                 "        (void)hb_block_alloc(hb_nr, &s_nrt);",
             ]
-        k += [
-            "        __syncthreads();",
-            "        if (threadIdx.x == 0) {",
-            "            const int hb_ntot = s_nsp & 0xffff;" if packed else "            const int hb_ntot = s_nsp;",
-            "            s_base = hb_ntot ? atomicAdd(&p.ring_cnt[slot], hb_ntot) : 0;",
-        ]
+        k += ["        __syncthreads();", "        if (threadIdx.x == 0) {"]
+        if R > 1:
+            k += [f"            for (int r = 0; r < {R}; r++) s_pbase[r] = s_npart[r] ? atomicAdd(&p.ring_cnt[slot * {R} + r], s_npart[r]) : 0;"]
+        else:
+            k += [
+                "            const int hb_ntot = s_nsp & 0xffff;" if packed else "            const int hb_ntot = s_nsp;",
+                "            s_base = hb_ntot ? atomicAdd(&p.ring_cnt[slot], hb_ntot) : 0;",
+            ]
         if self.spikemon:
             k += [
                 "            const int hb_nmtot = s_nsp >> 16;" if packed else "            const int hb_nmtot = s_nspm;",
@@ -624,11 +653,11 @@ class NeuronKernel:
                 "            if (s_nrt && p.rate_cnt && step - p.rate_row0 >= 0 && step - p.rate_row0 < p.rate_rows)",
                 "                atomicAdd(&p.rate_cnt[step - p.rate_row0], s_nrt);",
             ]
-        k += [
-            "        }",
-            "        __syncthreads();",
-            "        int hb_o = s_base + hb_off;",
-        ]
+        k += ["        }", "        __syncthreads();"]
+        if R > 1:
+            k += [f"        hb_op{r} += s_pbase[{r}];" for r in range(R)]
+        else:
+            k.append("        int hb_o = s_base + hb_off;")
         if self.spikemon:
             k.append("        unsigned long long hb_om = s_mbase + (unsigned long long)hb_offm;")
         k += [
@@ -636,9 +665,16 @@ class NeuronKernel:
             f"        for (int q = 0; q < {NPT}; q++) {{",
             "            if (hb_condq[q]) {",
             f"                const int i = hb_i0 + q * {T};",
-            "                p.ring_spk[(size_t)slot * p.ring_stride + hb_o] = i;",
-            "                hb_o++;",
         ]
+        if R > 1:
+            for r in range(R):
+                k.append(f"                {'else ' if r else ''}if (i < {bounds[r + 1]}) "
+                         f"p.ring_spk[(size_t)slot * p.ring_stride + {bounds[r]} + hb_op{r}++] = i;")
+        else:
+            k += [
+                "                p.ring_spk[(size_t)slot * p.ring_stride + hb_o] = i;",
+                "                hb_o++;",
+            ]
         if self.spikemon:
             k += [
                 "                if (i >= p.mon_start && i < p.mon_stop) {",
@@ -649,7 +685,8 @@ class NeuronKernel:
         k += [
             "            }",
             "        }",
-            "        if (blockIdx.x == 0 && threadIdx.x == 0) p.ring_cnt[(int)((step + 1) % p.ring_slots)] = 0;",
+            "        if (blockIdx.x == 0 && threadIdx.x == 0) p.ring_cnt[(int)((step + 1) % p.ring_slots)] = 0;" if R == 1 else
+            f"        if (blockIdx.x == 0 && threadIdx.x < {R}) p.ring_cnt[(int)((step + 1) % p.ring_slots) * {R} + threadIdx.x] = 0;",
             "    }",
             "}",
         ]
@@ -671,7 +708,7 @@ class GatherKernel:
 
     def __init__(self, name, rank, n_seg, dmin, on_pre, syn_arrays, scalars, val_t, float_t, target_var,
                  n_tiles, tile, count_synapses=False, pairs=256, target_index=None, plastic=(),
-                 events_per_thread=None):
+                 events_per_thread=None, ring_parts=None):
         if rank >= MAX_RANK:
             raise NotImplementedError(f"more than {MAX_RANK} pathways into one group")
         if n_seg > MAX_SEG:
@@ -699,6 +736,10 @@ class GatherKernel:
         self.tile = tile
         self.pairs = pairs
         self.count_synapses = count_synapses
+        # the source's spike ring split in parts (n_parts, first part read, starts of the parts
+        # read): the spikes of part r of a slot are at slot * stride + start_r, counted in
+        # ring_cnt[slot * n_parts + r]; the gather reads the parts covering its source range
+        self.ring_parts = ring_parts or (1, 0, (0,))
         self.kernel_name = f"gather_{name}"
         self._build()
 
@@ -742,6 +783,9 @@ class GatherKernel:
         WN = GT * U
         PC = self.pairs
         NS = self.n_seg
+        RP, P0, PB = self.ring_parts
+        NR = len(PB)  # parts read: segment s of the pair enumeration is (delay s // NR, part s % NR)
+        NSR = NS * NR
         NT = self.n_tiles
         TT = self.tile
         VT = self.val_t
@@ -769,7 +813,7 @@ class GatherKernel:
             "    const long long step = p.clock[0] + p.step_off;",
             "    const int par = (int)(step % 3);  // staging buffer of this step",
             f"    {rec_type(VT)}* hb_st = ({rec_type(VT)}*)p.stg_rec + (size_t)par * p.stg_stride;",
-            f"    __shared__ int s_cum[{NS + 1}];",
+            f"    __shared__ int s_cum[{NSR + 1}];",
             f"    __shared__ unsigned int s_a[{PC}];",
             f"    __shared__ int s_o[{PC}];",
             f"    __shared__ unsigned short s_sg[{PC}];",
@@ -778,7 +822,7 @@ class GatherKernel:
             f"    __shared__ unsigned short s_pid[{WN}];",
         ]
         vb = 8 if VT in ("double", "long long") else 4
-        lds = WN * 2 + NT * 8 + (NS + 1) * 4 + PC * 10 + 1024
+        lds = WN * 2 + NT * 8 + (NSR + 1) * 4 + PC * 10 + 1024
         if lds > LDS_BYTES:
             raise NotImplementedError("too many target tiles for the gather's LDS")
         self.staged = GATHER_STAGE_RUNS and lds + WN * (8 + vb) <= LDS_BYTES
@@ -795,18 +839,28 @@ class GatherKernel:
             f"    for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
             "    {",
             "        int carry = 0;",
-            f"        for (int s0 = 0; s0 < {NS}; s0 += {GT}) {{",
+            f"        for (int s0 = 0; s0 < {NSR}; s0 += {GT}) {{",
             "            const int s = s0 + (int)threadIdx.x;",
-            f"            const long long ps_ = step - {self.dmin} - s;",
-            f"            const int n_ = s < {NS} && ps_ >= 0 ? p.ring_cnt[(int)(ps_ % p.ring_slots)] : 0;",
+        ]
+        if NR == 1 and RP == 1:
+            k += [
+                f"            const long long ps_ = step - {self.dmin} - s;",
+                f"            const int n_ = s < {NS} && ps_ >= 0 ? p.ring_cnt[(int)(ps_ % p.ring_slots)] : 0;",
+            ]
+        else:
+            k += [
+                f"            const long long ps_ = step - {self.dmin} - s / {NR};",
+                f"            const int n_ = s < {NSR} && ps_ >= 0 ? p.ring_cnt[(int)(ps_ % p.ring_slots) * {RP} + {P0} + s % {NR}] : 0;",
+            ]
+        k += [
             "            int tot;",
             "            const int x = hb_block_exscan(n_, s_sh, &tot) + carry;",
-            f"            if (s < {NS}) s_cum[s + 1] = x + n_;",
+            f"            if (s < {NSR}) s_cum[s + 1] = x + n_;",
             "            carry += tot;",
             "        }",
             "    }",
             "    __syncthreads();",
-            f"    const int P = s_cum[{NS}];",
+            f"    const int P = s_cum[{NSR}];",
             f"    const int n_units = (P + {PC - 1}) / {PC};",
             "    unsigned long long hb_ev = 0ull;",
             "    // one (delay, spike) pair per thread of a unit: its delay segment and source (read",
@@ -817,11 +871,24 @@ class GatherKernel:
             f"        const int q_ = (c_) * {PC} + (int)threadIdx.x; \\",
             "        src_ = -1; seg_ = 0; \\",
             f"        if ((int)threadIdx.x < {PC} && q_ < P) {{ \\",
-            f"            int lo = 0, hi = {NS}; \\",
+            f"            int lo = 0, hi = {NSR}; \\",
             "            while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (s_cum[m] <= q_) lo = m; else hi = m; } \\",
-            "            seg_ = lo; \\",
-            f"            const int slot = (int)((step - {self.dmin} - lo) % p.ring_slots); \\",
-            "            src_ = p.ring_spk[(size_t)slot * p.ring_stride + (q_ - s_cum[lo])]; } }",
+        ]
+        if NR == 1 and RP == 1:
+            k += [
+                "            seg_ = lo; \\",
+                f"            const int slot = (int)((step - {self.dmin} - lo) % p.ring_slots); \\",
+                "            src_ = p.ring_spk[(size_t)slot * p.ring_stride + (q_ - s_cum[lo])]; } }",
+            ]
+        else:
+            start = " : ".join(f"hb_j == {j} ? {b}" for j, b in enumerate(PB[:-1])) + (" : " if NR > 1 else "") + str(PB[-1])
+            k += [
+                f"            seg_ = lo / {NR}; \\",
+                f"            const int hb_j = lo % {NR}; \\",
+                f"            const int slot = (int)((step - {self.dmin} - seg_) % p.ring_slots); \\",
+                f"            src_ = p.ring_spk[(size_t)slot * p.ring_stride + ({start}) + (q_ - s_cum[lo])]; (void)hb_j; }} }}",
+            ]
+        k += [
             "#define HB_PAIR_SEG(seg_, src_, a_, len_) { \\",
             "        a_ = 0u; len_ = 0u; \\",
             "        if (src_ >= p.src0 && src_ < p.src1) { \\",
@@ -1011,8 +1078,9 @@ class PostPathKernel:
 
     BLOCK = 256
 
-    def __init__(self, name, statements, syn_arrays, plastic, scalars, float_t, delay_steps):
+    def __init__(self, name, statements, syn_arrays, plastic, scalars, float_t, delay_steps, ring_parts=None):
         self.name = name
+        self.ring_parts = tuple(ring_parts or (0,))  # part starts of the target's spike ring
         self.statements = statements
         self.syn_arrays = syn_arrays
         self.plastic = set(plastic)
@@ -1051,9 +1119,27 @@ class PostPathKernel:
             f"    const long long ps_ = step - {self.delay};",
             "    if (ps_ < 0) return;",
             "    const int slot = (int)(ps_ % p.ring_slots);",
-            "    const int P = p.ring_cnt[slot];",
-            "    for (int c = blockIdx.x; c < P; c += gridDim.x) {",
-            "        const int j = p.ring_spk[(size_t)slot * p.ring_stride + c];",
+        ]
+        R = len(self.ring_parts)
+        if R == 1:
+            k += [
+                "    const int P = p.ring_cnt[slot];",
+                "    for (int c = blockIdx.x; c < P; c += gridDim.x) {",
+                "        const int j = p.ring_spk[(size_t)slot * p.ring_stride + c];",
+            ]
+        else:
+            # the parts' spikes one after the other: part r holds spikes [c_r, c_{r+1}) of the slot
+            k += ["    const int hb_c0 = 0;"]
+            k += [f"    const int hb_c{r + 1} = hb_c{r} + p.ring_cnt[slot * {R} + {r}];" for r in range(R)]
+            pos = "c - hb_c0 + " + str(self.ring_parts[0])
+            for r in range(1, R):
+                pos = f"c >= hb_c{r} ? c - hb_c{r} + {self.ring_parts[r]} : {pos}"
+            k += [
+                f"    const int P = hb_c{R};",
+                "    for (int c = blockIdx.x; c < P; c += gridDim.x) {",
+                f"        const int j = p.ring_spk[(size_t)slot * p.ring_stride + ({pos})];",
+            ]
+        k += [
             "        const unsigned int lo = p.post_ptr[j], hi = p.post_ptr[j + 1];",
             f"        for (unsigned int k_ = lo + threadIdx.x; k_ < hi; k_ += {B}) {{",
             "            const unsigned int e = p.post_syn[k_];",

@@ -745,8 +745,16 @@ class Network:
             dmax = max([s.csr["dmin"] + s.csr["n_seg"] - 1 for s in paths if s.source is g]
                        + [s.post_delay_steps for s in paths if s.target is g and s.post_spec is not None] + [0])
             g.ring_slots = dmax + 2
+            # the ring of a neuron group is split at the bounds of its pathways' source ranges,
+            # so that each gather enumerates only spikes of its own range
+            cuts = {0}
+            if isinstance(g, NeuronGroup):
+                for s in paths:
+                    if s.source is g:
+                        cuts.update(b for b in s.src_range if 0 < b < g.N)
+            g.ring_parts = tuple(sorted(cuts))
             g.ring_spk = zeros(g.ring_slots * g.N, "int")
-            g.ring_cnt = zeros(g.ring_slots, "int")
+            g.ring_cnt = zeros(g.ring_slots * len(g.ring_parts), "int")
             if isinstance(g, NeuronGroup):
                 g._dev = {n: dev(g._host[n], ct) for n, ct in g.vars.items()}
         # incoming pathways -> buckets
@@ -938,7 +946,8 @@ class Network:
                                 statemon=g.stmon.variables if g.stmon else None,
                                 uses_refractory=g.uses_refractory,
                                 statemon_when=g.stmon.when if g.stmon else "start", tile=g.block, npt=g.npt,
-                                ratemon=g.rmon.range if g.rmon else None, waves_per_eu=waves)
+                                ratemon=g.rmon.range if g.rmon else None, waves_per_eu=waves,
+                                ring_parts=g.ring_parts)
 
         # two workgroups of 1024 threads per CU need at most 64 VGPRs: keep that build only
         # if the compiler fits it without spilling
@@ -1148,6 +1157,17 @@ class Network:
         s.launch_index = len(self._launches)
         self._add_launch(s.fn, s.grid, s.kernel.block, s.args, s.struct_cls)
 
+    @staticmethod
+    def _ring_parts_read(s):
+        """(parts of the source's ring, first part read, starts of the parts read) for the
+        gather of s: the parts that cover its source range."""
+        parts = s.source.ring_parts
+        bounds = list(parts) + [s.source.N]
+        read = [r for r in range(len(parts)) if bounds[r] >= s.src_range[0] and bounds[r + 1] <= s.src_range[1]]
+        if not read:  # an empty source range: every part (the gather's range check skips them all)
+            read = list(range(len(parts)))
+        return (len(parts), read[0], tuple(parts[r] for r in read))
+
     def _gather_setup(self, s, events_per_thread=None):
         """The gather kernel of pathway s (not compiled), its synapse records and its
         argument struct (s.kernel, s.args, s.struct_cls, s.grid)."""
@@ -1160,7 +1180,7 @@ class Network:
         K = GatherKernel(s.name, rank, s.csr["n_seg"], s.csr["dmin"], s.spec.vector, types_syn, sc, g.val_t,
                          g.float_t, s.target_var, g.n_tiles, g.tile, count_synapses=s.count_events,
                          pairs=s.pairs, target_index=s.target_index, plastic=s.plastic,
-                         events_per_thread=events_per_thread)
+                         events_per_thread=events_per_thread, ring_parts=self._ring_parts_read(s))
         s.kernel = K
         s.csr["rec"] = self._synapse_records(s, K.rec_arrays)
         if s.csr["rec"].numel() != max(1, s.csr["n_syn"]) * K.rec_words and s.csr["n_syn"] > 0:
@@ -1242,7 +1262,7 @@ class Network:
         host.update(getattr(s, "namespace", {}) or {})
         sc = scalar_values(s.post_spec, host, s.target.float_t)
         K = PostPathKernel(s.name, s.post_spec.vector, s.syn_ctypes(), s.plastic, sc, s.target.float_t,
-                           s.post_delay_steps)
+                           s.post_delay_steps, ring_parts=s.target.ring_parts)
         s.post_kernel = K
         s.post_fn = self.device.function(K.source, K.kernel_name)
         cls = K.struct.ctypes_struct()

@@ -236,3 +236,43 @@ def test_network_fuse_options():
         Network(fuse={"gather_blocks": 128})
     with pytest.raises(ValueError):
         Network(fuse={"gather_blocks": [0, 4]})
+
+
+def test_split_spike_ring_kernels_compile_for_gfx950():
+    """A group whose pathways read parts of it (exc/inh) keeps one spike list per part of
+    its ring: the step kernel compacts spikes per part, each gather enumerates only the
+    parts of its source range, and an on_post pathway reads every part."""
+    from brian2_amd.kernels import FusedStepKernel, PostPathKernel, fold_capacity
+
+    m = Model.load("cuba_synw")
+    N = 5000
+    G = NeuronGroup(N, m, float_t="double")
+    cap = fold_capacity(1024, "float", 1, wgs_per_cu=2)
+    K = NeuronKernel("g", N, G.vars, "double", G.su, G.th, G.rs, "fused", _scalars(G),
+                     [(0, "ge", "+="), (1, "gi", "+=")], cap, "float", spikemon=(0, N), tile=1024, waves_per_eu=8,
+                     ring_parts=(0, 4000))
+    assert "s_npart" in K.source and "ring_cnt[slot * 2 + r]" in K.source
+    nt = -(-N // 1024)
+    gs = [GatherKernel(f"p{r}", r, 50, 1, m.pathways[pw].vector, {"w": "float"},
+                       scalar_values(m.pathways[pw], {"dt": 1e-4}, "double"), "float", "double", tv, nt, 1024,
+                       events_per_thread=3, ring_parts=(2, r, ((0, 4000)[r],)))
+          for r, (pw, tv) in enumerate((("exc", "ge"), ("inh", "gi")))]
+    assert "* 2 + 1 + s % 1]" in gs[1].source
+    both = GatherKernel("pall", 0, 5, 1, m.pathways["exc"].vector, {"w": "float"},
+                        scalar_values(m.pathways["exc"], {"dt": 1e-4}, "double"), "float", "double", "ge", nt, 1024,
+                        ring_parts=(2, 0, (0, 4000)))
+    s = Model.load("stdp")
+    H = NeuronGroup(20, s)
+    from brian2_amd.runtime import SpikeGeneratorGroup, Synapses
+
+    S = Synapses(SpikeGeneratorGroup(10, [0], timebins=[0]), H, s.pathways["pre"], on_post=s.pathways["post"])
+    pk = PostPathKernel("stdp", S.post_spec.vector, {n: "double" for n in S.syn_vars}, S.plastic,
+                        scalar_values(S.post_spec, {"dt": 1e-4}, "double"), "double", 0, ring_parts=(0, 7, 12))
+    assert "hb_c3" in pk.source
+    F = FusedStepKernel(K, gs, [7, 3], nt)
+    for src in (K.source, gs[0].source, both.source, pk.source, F.source):
+        _no_float_atomics(src)
+        _compile(src)
+    with pytest.raises(ValueError):
+        NeuronKernel("g", N, G.vars, "double", G.su, G.th, G.rs, "fused", _scalars(G), [], cap, "float",
+                     tile=1024, ring_parts=(10, 4000))

@@ -54,3 +54,41 @@ def test_stdp_store_restore_and_pieces():
         assert np.array_equal(x, y)
     sel = z["step"] < 5000
     assert np.array_equal(mon.i, z["i"][sel]) and np.array_equal(mon.step, z["step"][sel])
+
+
+def _zero_ge_spec():
+    """on_pre='ge += w' as Brian's statements (a pathway that adds w to ge)."""
+    from brian2_amd.kernels import CodeSpec
+
+    variables = {"ge": {"kind": "ArrayVariable", "dtype": "float64", "owner": "target"},
+                 "w": {"kind": "DynamicArrayVariable", "dtype": "float64", "owner": "synapses"},
+                 "_postsynaptic_idx": {"kind": "DynamicArrayVariable", "dtype": "int32", "owner": "synapses"}}
+    indices = {"ge": "_postsynaptic_idx", "w": "_idx", "_postsynaptic_idx": "_idx"}
+    return CodeSpec({"name": "zero_pre_codeobject", "template": "synapses", "owner": "synapses",
+                     "blocks": {"None": {"scalar": [], "vector": [{"var": "ge", "op": "+=", "expr": "w",
+                                                                   "dtype": "float64"}],
+                                         "conditional_write": {}}},
+                     "variables": variables, "indices": indices})
+
+
+def test_stdp_with_a_split_spike_ring():
+    """A pathway from neurons[0:8] (zero weights, delay 1 step, applied after the STDP
+    pathway: ge + 0.0 == ge) splits the neurons' spike ring in two parts: the on_post
+    pathway then reads both parts.  Results equal Brian's STDP run."""
+    from brian2_amd import Synapses
+
+    z, meta = O.load_golden("stdp")
+    net, G, S, mon = _stdp_network(meta)
+    Z = Synapses(G[0:8], G, _zero_ge_spec(), name="zero")
+    pre = np.repeat(np.arange(8), meta["n_out"])
+    post = np.tile(np.arange(meta["n_out"]), 8)
+    Z.connect(pre, post)
+    Z.set("w", 0.0)
+    Z.set_delay(1e-4)
+    net.add(Z)
+    net.run(meta["n_steps"])
+    assert G.ring_parts == (0, 8)
+    assert np.array_equal(mon.i, z["i"]) and np.array_equal(mon.step, z["step"])
+    assert np.array_equal(np.stack([G.get("v"), G.get("ge")]), z["final"])
+    for k, n in enumerate(SYN_VARS):
+        assert np.array_equal(S.get(n), z["syn"][k]), n
