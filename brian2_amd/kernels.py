@@ -505,12 +505,16 @@ class NeuronKernel:
         k.append("    }")
         if self.fold:
             fk, U, W = fold[1], fold[2], fold[3]
+            # hb_tot is the same for every thread of the workgroup: a tile without events skips
+            # the fold (its barriers included)
+            k.append("    if (hb_tot > 0) {")
             k += fk
             apply = ["switch ((int)((hb_k >> 44) & 15)) {"]
             for rank, var, op in self.fold:
                 apply.append(f"case {rank}: hb_a_{var}[q] {op} hb_v; break;")
             apply += ["default: break;", "}"]
             k += _fold_passes(T, U, W, self.cap, self.val_t, apply, NPT)
+            k += ["    } else {", "    __syncthreads();  // the workgroup's counters, cleared at the start", "    }"]
         # ---- per-neuron body ------------------------------------------------------
         k.append("    #pragma unroll")
         k.append(f"    for (int q = 0; q < {NPT}; q++) {{")
