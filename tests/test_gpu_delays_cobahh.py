@@ -99,3 +99,30 @@ def test_cobahh_fp32_error_vs_fp64():
     n32 = len(c["mon"].i)
     print(f"COBAHH fp32 vs fp64 over 1000 steps: max |dv| = {err:.3e} V, spikes {n32} vs {n64}")
     assert abs(n32 - n64) <= 0.05 * n64 + 5
+
+
+@pytest.mark.parametrize("shape", [{}, {"tile": 1024}], ids=["default", "tile1024-fused"])
+def test_synw_delays_with_a_whole_group_pathway(shape):
+    """A third pathway from the whole group (zero weights into ge, delays 1-3 steps,
+    last in the fold order: ge + 0.0 == ge) reads both parts of the group's spike ring
+    (the exc and inh pathways read one each); results equal Brian's."""
+    from brian2_amd import Synapses, synth
+    from test_gpu_stdp import _zero_ge_spec
+
+    z, meta = O.load_golden("cuba_delay")
+    c = synth.cuba_synw_network(N=meta["N"], K=meta["K"], dmin=meta["dmin"], dmax=meta["dmax"], seed=meta["seed"],
+                                ginh=meta["ginh"], record=[0, 1, 2, 3], network_options=shape)
+    G = c["G"]
+    rng = np.random.default_rng(3)
+    n = 20000
+    Z = Synapses(G, G, _zero_ge_spec(), name="zzz")
+    Z.connect(rng.integers(0, G.N, n), rng.integers(0, G.N, n))
+    Z.set("w", 0.0)
+    Z.set_delay(rng.integers(1, 4, n) * 1e-4)
+    c["net"].add(Z)
+    print(f"cuba_delay golden + whole-group pathway, launch shape {shape}", flush=True)
+    c["net"].run(meta["n_steps"])
+    assert len(G.ring_parts) == 2 and len(Z.kernel.ring_parts[2]) == 2
+    assert np.array_equal(c["mon"].i, z["i"]) and np.array_equal(c["mon"].step, z["step"])
+    final = np.stack([G.get(n_) for n_ in ("v", "ge", "gi")])
+    assert np.array_equal(final, z["final"])
