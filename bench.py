@@ -198,8 +198,8 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the network would shard)", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: there is no multi-GPU path (a network would shard by "
+              "neuron range, exchanging each step's spike lists)", file=sys.stderr)
         sys.exit(2)
 
     import torch
