@@ -126,3 +126,23 @@ def test_synw_delays_with_a_whole_group_pathway(shape):
     assert np.array_equal(c["mon"].i, z["i"]) and np.array_equal(c["mon"].step, z["step"])
     final = np.stack([G.get(n_) for n_ in ("v", "ge", "gi")])
     assert np.array_equal(final, z["final"])
+
+
+def test_fused_and_separate_launches_agree_in_fp32():
+    """fp32 state (the reported-error mode): the fused launch and separate launches run
+    the same bodies in the same fold order, so their spikes and final state are
+    identical bit for bit."""
+    from brian2_amd import synth
+
+    z, meta = O.load_golden("cuba_delay")
+    out = []
+    for opt in ({"tile": 1024}, {"tile": 1024, "fuse": False}):
+        c = synth.cuba_synw_network(N=meta["N"], K=meta["K"], dmin=meta["dmin"], dmax=meta["dmax"], seed=meta["seed"],
+                                    ginh=meta["ginh"], float_t="float", network_options=opt)
+        c["net"].run(3000)
+        assert (c["G"].fused is not None) == opt.get("fuse", True)
+        out.append((c["mon"].i.copy(), c["mon"].step.copy(), [c["G"].get(n) for n in ("v", "ge", "gi")]))
+    (i0, s0, f0), (i1, s1, f1) = out
+    assert len(i0) > 100
+    assert np.array_equal(i0, i1) and np.array_equal(s0, s1)
+    assert all(np.array_equal(a, b) for a, b in zip(f0, f1))
