@@ -3,29 +3,42 @@
 Workload (the 1M-neuron configuration): CUBA LIF, N = 1,000,000 (800k exc /
 200k inh), fixed in-degree 1000 drawn without replacement (1e9 synapses,
 int32 indices, fp32 weights), per-synapse delays U{1..50} steps, dt = 0.1 ms,
-fp64 state (bit-exact verification mode), 500 ms simulated.  One "step" is
-one simulation time step of the whole network (state update, threshold, spike
-queue, propagation).
+fp64 state (bit-exact verification mode).
 
-Measurement: after W warm-up steps the network state is stored on the device,
-K steps are timed with hipGraph replay (the value), the state is restored and
-the same K steps are run again one launch at a time with HIP events around
-each launch on the plan's stream (per-kernel durations and the roofline; the
-simulation is deterministic, so both passes process identical events).
+One bench step = 100 simulation time steps (10 ms simulated), replayed as one
+hipGraph of 100 step launches.  The network alternates between bursts and
+quiet epochs of a few ms each, so one simulation step is not a representative
+unit of work; 100 of them are.  Before the timed region the network settles
+for --settle simulation steps (default 2000, far beyond the 50-step delay
+window and the start-up transient) plus W warm-up bench steps, all untimed.
+Then exactly K bench steps are timed (K x 100 simulation steps), bracketed by
+torch.cuda.synchronize().
 
-Second leg: the batched sweep (1024 CUBA networks, N ~ U{500..8000}, swept
-excitatory weight): networks·simulated-seconds per second of the persistent
-batched kernel vs the plain per-network path on a sample of the same networks,
-with the sample's spikes checked identical.
+Roofline: the step kernel of this workload is one launch per simulation step
+(`fused_neurongroup`: the group's tiles and both pathways' gathers).  Its
+average duration is the device time of the timed region, taken from HIP
+events recorded on the plan's stream around the graph launches (host round
+trips between chunks excluded), divided by the launches; its algorithmic
+bytes per launch are N x 58 B + 20 B per spike + 9 B per synaptic event
+(fp64 state read and written, lastspike read, not_refractory read and
+written; per spike the ring entry, the monitor record and lastspike; per
+event the target index, the fp32 weight and the delay code), from the events
+and spikes counted in that region.  A second pass replays the same steps
+from the stored state one launch at a time with HIP events around each launch
+(per-kernel figures; identical events and spikes are checked).
+
+Further legs: the batched sweep (1024 CUBA networks, fp64 and fp32) against
+the plain per-network path; the differentiable LIF network (forward +
+adjoint backward); the fp32 mode's error against fp64 per configuration; the
+CPU baseline (the C restatement, one thread and OpenMP) on a bounded sample.
 
 Prints one JSON line on stdout; progress goes to stderr.
 
-Third leg: the differentiable LIF network (256 trials of 200 ms): forward and
-exact adjoint backward, trials x simulated seconds per second.
-
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-fuse] [--no-sweep] [--no-adjoint] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--settle S] [--no-fuse] [--float float]
+                    [--no-sweep] [--no-adjoint] [--no-fp32-error] [--no-cpu-baseline] [--no-profile-pass]
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -42,9 +55,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n=20000, K=1000, warm=500, steps=5000):
-    """The oracle (plain C restatement, 1 thread) on a bounded sample of the same
-    workload: the same model, in-degree, delays and weights at N = n neurons."""
+def cpu_baseline(n=20000, K=1000, warm=1000, steps=2000):
+    """The C restatement of the step (the oracle) on a bounded sample of the same
+    workload: the same model, in-degree, delays and weights at N = n neurons, one
+    thread and then every core of the host (OpenMP)."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import numpy as np
 
@@ -53,18 +67,28 @@ def cpu_baseline(n=20000, K=1000, warm=500, steps=5000):
     meta = dict(N=n, NE=n * 4 // 5, K=K, taum=0.02, taue=0.005, taui=0.01, El=-0.049, Vt=-0.05, Vr=-0.06,
                 refractory=0.005, dt=DT, dmin=1, dmax=50, seed=1234, we_base=1.62e-3 * 80.0 / K,
                 wi_base=-9e-3 * 80.0 / K * 0.4)
-    G, paths = O.build_cuba_delay(meta)
-    for p in paths:
-        p.counts = np.zeros(len(p.pre), np.int32)
-    O.run([G], paths, meta["dt"], warm, mon_group=None)
-    before = sum(int(p.counts.sum()) for p in paths)
-    t0 = time.perf_counter()
-    O.run([G], paths, meta["dt"], steps, start_step=warm, mon_group=None)
-    wall = time.perf_counter() - t0
-    events = sum(int(p.counts.sum()) for p in paths) - before
-    return dict(value=events / wall, unit="events/s", cores=1, kind="port",
-                sample=f"oracle (C restatement, 1 thread): same model at N={n}, in-degree {K}, delays 1-50, "
-                       f"{steps} steps after {warm} warm-up steps; {events} events in {wall:.1f} s")
+    ncpu = O.cpu_threads()
+    out = {}
+    for threads in (1, ncpu):
+        G, paths = O.build_cuba_delay(meta)
+        for p in paths:
+            p.counts = np.zeros(len(p.pre), np.int32)
+        O.run([G], paths, meta["dt"], warm, mon_group=None, threads=threads)
+        before = sum(int(p.counts.sum()) for p in paths)
+        t0 = time.perf_counter()
+        O.run([G], paths, meta["dt"], steps, start_step=warm, mon_group=None, threads=threads)
+        wall = time.perf_counter() - t0
+        events = sum(int(p.counts.sum()) for p in paths) - before
+        out[threads] = (events / wall, events, wall, G.state.copy())
+        log(f"  cpu baseline, {threads} thread(s): {events} events in {wall:.2f} s -> {events / wall:.3e} events/s")
+    same = bool(np.array_equal(out[1][3], out[ncpu][3]))
+    v1, vn = out[1][0], out[ncpu][0]
+    return dict(value=vn, unit="events/s", cores=ncpu, kind="port",
+                single_thread=v1, threads_used=ncpu, host_cpu_count=os.cpu_count(),
+                sample=f"plain CPU path (restatement): the oracle (C, -O2, no FMA contraction) on the same model at "
+                       f"N={n}, in-degree {K}, delays 1-50, {steps} steps after {warm} settle steps; "
+                       f"{out[ncpu][1]} events in {out[ncpu][2]:.1f} s on {ncpu} OpenMP threads, "
+                       f"{out[1][1]} in {out[1][2]:.1f} s on one; final state identical: {same}")
 
 
 def kernel_bytes(kind, n, events, spikes, fbytes):
@@ -80,7 +104,7 @@ def kernel_bytes(kind, n, events, spikes, fbytes):
     return step + gather if kind == "fused" else gather
 
 
-def sweep_bench(n_nets=1024, steps=10000, sample=16, sample_steps=1000, float_t="double"):
+def sweep_bench(n_nets=1024, steps=10000, sample=16, sample_steps=1000, float_t="double", ref_counts=None):
     """Batched sweep vs the plain per-network path (nets·sim-s/s)."""
     import numpy as np
     import torch
@@ -92,8 +116,8 @@ def sweep_bench(n_nets=1024, steps=10000, sample=16, sample_steps=1000, float_t=
     t0 = time.perf_counter()
     B.build()
     best, res = B.autotune(trial_steps=200)
-    log(f"sweep: {n_nets} networks ({sum(n.N for n in nets)} neurons, {B.n_syn} synapses) built and tuned in "
-        f"{time.perf_counter() - t0:.1f} s: block {best[0]}, {best[1]} steps per launch")
+    log(f"sweep ({float_t}): {n_nets} networks ({sum(n.N for n in nets)} neurons, {B.n_syn} synapses) built and "
+        f"tuned in {time.perf_counter() - t0:.1f} s: block {best[0]}, {best[1]} steps per launch")
     steps -= steps % B.steps_per_launch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -101,7 +125,8 @@ def sweep_bench(n_nets=1024, steps=10000, sample=16, sample_steps=1000, float_t=
     torch.cuda.synchronize()
     wall_b = time.perf_counter() - t0
     rate_b = n_nets * steps * DT / wall_b
-    log(f"sweep: batched {steps} steps in {wall_b:.2f} s -> {rate_b:.1f} nets*sim-s/s")
+    log(f"sweep ({float_t}): batched {steps} steps in {wall_b:.2f} s -> {rate_b:.1f} nets*sim-s/s")
+    counts = B.spk_n.cpu().numpy().astype(np.int64)
     idx = np.unique(np.linspace(0, n_nets - 1, sample).astype(int))
     wall_p, sim_p, same = 0.0, 0.0, True
     for k in idx:
@@ -120,14 +145,24 @@ def sweep_bench(n_nets=1024, steps=10000, sample=16, sample_steps=1000, float_t=
         same &= bool(np.array_equal(i[sel][o], c["mon"].i) and np.array_equal(s[sel][o], c["mon"].step))
         del c
     rate_p = sim_p / wall_p
-    log(f"sweep: plain path {len(idx)} networks x {sample_steps} steps in {wall_p:.2f} s -> {rate_p:.2f} "
-        f"nets*sim-s/s; spikes identical: {same}")
-    return {"metric": "nets*sim-s/s", "value": rate_b, "unit": "networks x simulated s / s",
-            "plain_value": rate_p, "vs_plain": rate_b / rate_p, "spikes_identical": same,
-            "dtype": "f64" if float_t == "double" else "f32", "n_nets": n_nets, "steps": steps,
-            "block": best[0], "steps_per_launch": best[1],
-            "plain_sample": f"{len(idx)} of the {n_nets} networks (evenly spaced), {sample_steps} steps each, "
-                            f"one Network per network, hipGraph replay of 100 steps"}
+    log(f"sweep ({float_t}): plain path {len(idx)} networks x {sample_steps} steps in {wall_p:.2f} s -> "
+        f"{rate_p:.2f} nets*sim-s/s; spikes identical: {same}")
+    out = {"metric": "nets*sim-s/s", "value": rate_b, "unit": "networks x simulated s / s",
+           "plain_value": rate_p, "vs_plain": rate_b / rate_p, "spikes_identical": same,
+           "dtype": "f64" if float_t == "double" else "f32", "n_nets": n_nets, "steps": steps,
+           "n_neurons": int(sum(n.N for n in nets)), "n_synapses": int(B.n_syn),
+           "block": best[0], "steps_per_launch": best[1],
+           "plain_sample": f"{len(idx)} of the {n_nets} networks (evenly spaced), {sample_steps} steps each, "
+                           f"one Network per network, hipGraph replay of 100 steps"}
+    if ref_counts is not None:
+        d = counts - ref_counts
+        out["fp32_error"] = {"vs": "the fp64 sweep (same networks, same steps)",
+                             "spikes_f64": int(ref_counts.sum()), "spikes_f32": int(counts.sum()),
+                             "total_rel_delta": float(d.sum() / max(1, ref_counts.sum())),
+                             "mean_abs_rel_delta_per_net": float(np.mean(np.abs(d) / np.maximum(1, ref_counts))),
+                             "nets_with_equal_counts": int((d == 0).sum())}
+    del B
+    return out, counts
 
 
 def adjoint_bench(trials=256):
@@ -158,6 +193,73 @@ def adjoint_bench(trials=256):
             "backward_ms": t_b * 1e3, "ckpt_every": net.C, "loss": loss, "dtype": "f64 state, f32 weights"}
 
 
+def fp32_error_pair(make, n_steps, chunk, variables):
+    """Runs one configuration in fp64 and in fp32 side by side, chunk by chunk.
+    Returns the spike counts of both, the first step at which the spike lists
+    differ (None: never), and per state variable the largest |x32 - x64| at a
+    chunk boundary before that step, absolute and relative to max |x64|."""
+    import numpy as np
+
+    c64, c32 = make("double"), make("float")
+    err = {v: [0.0, 0.0] for v in variables}
+    first_div = None
+    done = 0
+    while done < n_steps:
+        k = min(chunk, n_steps - done)
+        c64["net"].run(k)
+        c32["net"].run(k)
+        done += k
+        if first_div is None:
+            i64, s64 = c64["mon"].i, c64["mon"].step
+            i32, s32 = c32["mon"].i, c32["mon"].step
+            if not (np.array_equal(i64, i32) and np.array_equal(s64, s32)):
+                m = min(len(s64), len(s32))
+                bad = np.nonzero((i64[:m] != i32[:m]) | (s64[:m] != s32[:m]))[0]
+                first_div = int(min(s64[bad[0]], s32[bad[0]]) if len(bad) else
+                                (s64[m] if len(s64) > m else s32[m]))
+                continue
+            for v in variables:
+                a, b = c64["G"].get(v), c32["G"].get(v).astype(np.float64)
+                scale = float(np.max(np.abs(a))) or 1.0
+                e = float(np.max(np.abs(a - b)))
+                err[v][0] = max(err[v][0], e)
+                err[v][1] = max(err[v][1], e / scale)
+    n64, n32 = len(c64["mon"].i), len(c32["mon"].i)
+    del c64, c32
+    return {"steps": n_steps, "spikes_f64": n64, "spikes_f32": n32,
+            "spike_count_rel_delta": (n32 - n64) / max(1, n64),
+            "first_divergent_step": first_div, "compared_every": chunk,
+            "max_abs_err_before_divergence": {v: err[v][0] for v in variables},
+            "max_rel_err_before_divergence": {v: err[v][1] for v in variables}}
+
+
+def fp32_errors():
+    """The fp32 mode's error against fp64 (the verification mode) per configuration."""
+    import gc
+
+    import torch
+
+    from brian2_amd import synth
+
+    out = {}
+    t0 = time.perf_counter()
+    out["cuba4000"] = fp32_error_pair(lambda ft: synth.cuba_network(N=4000, float_t=ft), 10000, 100,
+                                      ["v", "ge", "gi"])
+    out["cobahh4000"] = fp32_error_pair(lambda ft: synth.cobahh_network(N=4000, float_t=ft), 10000, 100,
+                                        ["v", "m", "n", "h", "ge", "gi"])
+    out["cuba_1m_at_100k"] = fp32_error_pair(
+        lambda ft: synth.cuba_synw_network(N=100_000, K=1000, float_t=ft), 5000, 250, ["v", "ge", "gi"])
+    out["cuba_1m_at_100k"]["note"] = ("the 1M-neuron configuration's model, in-degree, delays and weights at "
+                                      "N = 100,000")
+    gc.collect()
+    torch.cuda.empty_cache()
+    log(f"fp32 error record: {time.perf_counter() - t0:.1f} s")
+    for k, v in out.items():
+        log(f"  {k}: spikes {v['spikes_f64']} (f64) vs {v['spikes_f32']} (f32), first divergent step "
+            f"{v['first_divergent_step']}, max rel err before it {v['max_rel_err_before_divergence']}")
+    return out
+
+
 def copy_calibration(nbytes=256 << 20, reps=3):
     """A plain device-to-device copy of known size (hipMemcpyAsync), so that a
     counter run (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) can scale the counters
@@ -179,20 +281,37 @@ def copy_calibration(nbytes=256 << 20, reps=3):
     log(f"copy calibration: {reps} device copies of {nbytes} bytes")
 
 
+def measured_traffic(kernel, sha1, workload):
+    """HBM bytes per launch of `kernel` from the committed counter runs, only when
+    they measured this very kernel source (sha1) on this workload; else None."""
+    tfile = os.path.join(REPO, "profiles", "traffic_cuba1m.json")
+    if not os.path.exists(tfile):
+        return None
+    with open(tfile) as f:
+        t = json.load(f)
+    e = t.get(kernel)
+    if not e or e.get("kernel_sha1") != sha1 or e.get("workload") != workload:
+        return None
+    return {k: e[k] for k in ("fetch_bytes", "write_bytes", "bytes", "dispatches", "kernel_sha1") if k in e}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=50, help="timed bench steps (each 100 simulation steps)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed bench steps after the settle")
+    ap.add_argument("--settle", type=int, default=2000, help="untimed simulation steps before the warm-up")
+    ap.add_argument("--sim-steps-per-step", type=int, default=100)
     ap.add_argument("--N", type=int, default=1_000_000)
     ap.add_argument("--K", type=int, default=1000)
-    ap.add_argument("--graph-steps", type=int, default=100)
     ap.add_argument("--float", dest="float_t", default="double", choices=["double", "float"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fuse", action="store_true",
                     help="the group step and each pathway's gather as separate launches (gathers on side streams)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-fp32-error", action="store_true")
+    ap.add_argument("--no-profile-pass", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with device copies of known size (for counter runs)")
     ap.add_argument("--sweep-steps", type=int, default=10000)
@@ -206,103 +325,137 @@ def main():
 
     from brian2_amd import synth
 
+    SPS = args.sim_steps_per_step
     log(f"device: {torch.cuda.get_device_name(0)}")
     if args.copy_calibration:
         copy_calibration()
     t0 = time.perf_counter()
-    c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=args.graph_steps,
+    c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=SPS,
                                 network_options={"fuse": not args.no_fuse})
     torch.cuda.synchronize()
     log(f"built network: {c['n_syn']} synapses in {time.perf_counter() - t0:.1f} s")
     net, mon = c["net"], c["mon"]
     t0 = time.perf_counter()
-    net.run(args.warmup)
+    net.run(args.settle + args.warmup * SPS)
     torch.cuda.synchronize()
-    log(f"warm-up {args.warmup} steps (includes kernel compilation, graph capture): {time.perf_counter() - t0:.2f} s")
+    log(f"settle {args.settle} + warm-up {args.warmup} x {SPS} simulation steps (includes kernel compilation, graph "
+        f"capture): {time.perf_counter() - t0:.2f} s")
     net.store()
+    step0 = net.t_step
 
+    K = args.steps
+    n_sim = K * SPS
     ev0 = net.n_events
     spk0 = int(mon.mon_n.item())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    net.run(args.steps)
+    net.run(n_sim, timing=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    gpu_ms = net.last_run_gpu_ms
     events = net.n_events - ev0
     spikes = int(mon.mon_n.item()) - spk0
-    ms_per_step = wall / args.steps * 1e3
-    rate = spikes / args.N / (args.steps * DT)
-    log(f"timed {args.steps} steps: {wall:.3f} s, {ms_per_step * 1e3:.1f} us/step, {events} events, "
+    rate = spikes / args.N / (n_sim * DT)
+    log(f"timed {K} x {SPS} simulation steps: {wall:.3f} s wall, {gpu_ms:.2f} ms device time of the step launches, "
+        f"{wall / n_sim * 1e6:.1f} us/simulation step, {events} events ({events / n_sim:.3e} per simulation step), "
         f"{spikes} spikes ({rate:.1f} Hz)")
 
-    # the same steps again, one launch at a time, HIP events around each launch
-    net.restore()
-    ev_p0 = {s.name: s.n_events for s in net.paths}
-    prof = net.profile(args.steps)
-    net.run(0)
-    ev_p = {s.name: s.n_events - ev_p0[s.name] for s in net.paths}
-    same = sum(ev_p.values()) == events and int(mon.mon_n.item()) - spk0 == spikes
-    log(f"profiled pass: {sum(ev_p.values())} events, identical to the timed pass: {same}")
     fbytes = 8 if args.float_t == "double" else 4
+    G = c["G"]
+    fused = getattr(G, "fused", None) is not None
     kernels = []
-    for kind, name, ms in prof:
-        avg_us = ms / args.steps * 1e3
-        # a fused launch runs the group's step and the gathers of every pathway into it
-        ev = sum(ev_p[s.name] for s in net.paths if s.target.name == name) if kind == "fused" else ev_p.get(name, 0)
-        alg = kernel_bytes(kind, args.N, ev / args.steps, spikes / args.steps, fbytes)
-        kernels.append(dict(kernel=f"{kind}_{name}", avg_us=avg_us, alg_bytes=alg,
-                            gbs=alg / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0))
-    for k in kernels:
-        log(f"  {k['kernel']}: {k['avg_us']:.1f} us/launch, {k['alg_bytes'] / 1e6:.1f} MB algorithmic, "
-            f"{k['gbs']:.0f} GB/s")
-    dom = max(kernels, key=lambda k: k["avg_us"])
-    step_bytes = kernel_bytes("step", args.N, 0, spikes / args.steps, fbytes) + \
-        kernel_bytes("gather", 0, events / args.steps, 0, fbytes)
-    step_frac = step_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
-    log(f"whole step: {step_bytes / 1e6:.1f} MB algorithmic per step -> {step_frac * 100:.1f}% of HBM peak")
-
-    traffic = None
-    tfile = os.path.join(REPO, "profiles", "traffic_cuba1m.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get(dom["kernel"])
+    same = None
+    if not args.no_profile_pass:
+        # the same steps again, one launch at a time, HIP events around each launch
+        net.restore()
+        ev_p0 = {s.name: s.n_events for s in net.paths}
+        prof = net.profile(n_sim)
+        net.run(0)
+        ev_p = {s.name: s.n_events - ev_p0[s.name] for s in net.paths}
+        same = sum(ev_p.values()) == events and int(mon.mon_n.item()) - spk0 == spikes
+        log(f"profiled pass: {sum(ev_p.values())} events, identical to the timed pass: {same}")
+        for kind, name, ms in prof:
+            avg_us = ms / n_sim * 1e3
+            ev = sum(ev_p[s.name] for s in net.paths if s.target.name == name) if kind == "fused" else ev_p.get(name, 0)
+            alg = kernel_bytes(kind, args.N, ev / n_sim, spikes / n_sim, fbytes)
+            kernels.append(dict(kernel=f"{kind}_{name}", avg_us=avg_us, alg_bytes=alg,
+                                gbs=alg / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0))
+        for k in kernels:
+            log(f"  {k['kernel']}: {k['avg_us']:.1f} us/launch (one launch at a time), {k['alg_bytes'] / 1e6:.1f} MB "
+                f"algorithmic, {k['gbs']:.0f} GB/s")
+    step_bytes = kernel_bytes("step", args.N, 0, spikes / n_sim, fbytes) + \
+        kernel_bytes("gather", 0, events / n_sim, 0, fbytes)
+    if fused:
+        # one launch per simulation step: the timed region's device time per launch
+        dom_name = f"fused_{G.name}"
+        avg_us = gpu_ms * 1e3 / n_sim
+        alg = step_bytes
+        src_sha1 = hashlib.sha1(G.fused.source.encode()).hexdigest()
+        avg_src = "HIP events on the plan's stream around the timed region's graph launches / launches"
+    else:
+        dom = max(kernels, key=lambda k: k["avg_us"]) if kernels else None
+        dom_name = dom["kernel"] if dom else None
+        avg_us = dom["avg_us"] if dom else float("nan")
+        alg = dom["alg_bytes"] if dom else float("nan")
+        src_sha1 = None
+        avg_src = "HIP events around each launch of the profiled pass"
+    gbs = alg / (avg_us * 1e-6) / 1e9
+    step_frac = step_bytes / (gpu_ms * 1e-3 / n_sim) / (HBM_PEAK_GBS * 1e9)
+    log(f"roofline: {dom_name} {avg_us:.2f} us/launch, {alg / 1e6:.2f} MB algorithmic -> {gbs:.0f} GB/s "
+        f"({gbs / HBM_PEAK_GBS * 100:.1f}% of 8 TB/s)")
+    workload = f"cuba_synw N={args.N} K={args.K} {args.float_t} settle={args.settle + args.warmup * SPS} steps={n_sim}"
 
     out = {
         "metric": "synaptic events/s",
         "value": events / wall,
         "unit": "events/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wall / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.float_t == "double" else "f32",
         "data": "synthetic",
-        "config": {"workload": "CUBA LIF 1M neurons, fixed in-degree 1000 (1e9 synapses), delays 1-50 steps, "
-                               "fp32 weights, dt 0.1 ms, 500 ms simulated (the 1M-neuron configuration)",
-                   "N": args.N, "in_degree": args.K, "n_synapses": c["n_syn"], "sim_steps_timed": args.steps,
-                   "mean_rate_hz": rate, "spikes": spikes, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom["kernel"],
-                     "avg_launch_us": dom["avg_us"], "step_frac": step_frac},
-        "kernels": kernels,
+        "config": {"workload": f"CUBA LIF {args.N} neurons, fixed in-degree {args.K} ({c['n_syn']} synapses), delays "
+                               f"1-50 steps, fp32 weights, dt 0.1 ms (the 1M-neuron configuration); timed: {n_sim} "
+                               f"simulation steps ({n_sim * DT * 1e3:.0f} ms simulated) after {step0} untimed ones",
+                   "step": f"{SPS} simulation steps ({SPS * DT * 1e3:.0f} ms simulated), one hipGraph replay",
+                   "N": args.N, "in_degree": args.K, "n_synapses": c["n_syn"], "settle_steps": args.settle,
+                   "warmup_sim_steps": args.warmup * SPS, "sim_steps_timed": n_sim,
+                   "events_per_sim_step": events / n_sim, "mean_rate_hz": rate, "spikes": spikes,
+                   "us_per_sim_step": wall / n_sim * 1e6, "launches": "fused" if fused else "separate",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic(dom_name, src_sha1, workload),
+                     "kernel": dom_name, "avg_launch_us": avg_us, "alg_bytes_per_launch": alg,
+                     "avg_launch_source": avg_src, "kernel_sha1": src_sha1, "step_frac": step_frac,
+                     "workload_key": workload},
+        "kernels_profiled_pass": kernels,
         "profiled_pass_identical": same,
     }
-    del c, net, mon
-    if not args.no_sweep:
-        import gc
+    del c, net, mon, G
+    import gc
 
+    gc.collect()
+    torch.cuda.empty_cache()
+    if not args.no_sweep:
+        out["batched"], ref = sweep_bench(steps=args.sweep_steps)
         gc.collect()
         torch.cuda.empty_cache()
-        out["batched"] = sweep_bench(steps=args.sweep_steps)
+        out["batched"]["fp32"], _ = sweep_bench(steps=args.sweep_steps, float_t="float", ref_counts=ref)
+        gc.collect()
+        torch.cuda.empty_cache()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench()
+    if not args.no_fp32_error:
+        out["fp32_error"] = fp32_errors()
+        if "batched" in out:
+            out["fp32_error"]["sweep_1024"] = out["batched"]["fp32"].get("fp32_error")
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle on a bounded sample) ...")
+        log("cpu baseline (the restatement on a bounded sample) ...")
         out["cpu_baseline"] = cpu_baseline()
-        log(f"  {out['cpu_baseline']['value']:.3e} events/s")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -58,6 +58,7 @@ _SIGS = {
     "hb_plan_set_args": ([_vp, _i, _vp, _sz], _i),
     "hb_plan_set_lane": ([_vp, _i, _i], _i),
     "hb_plan_run": ([_vp, _vp, _i64, _i], _i),
+    "hb_plan_timing": ([_vp, _i, _P(_dbl), _P(_i64)], _i),
     "hb_plan_destroy": ([_vp], _i),
     "hb_plan_profile": ([_vp, _vp, _i64, _P(ctypes.c_float)], _i),
     "hb_spikegen_function": ([_P(_vp)], _i),

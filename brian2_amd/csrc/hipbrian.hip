@@ -301,21 +301,39 @@ struct hb_launch_rec {
     int lane = 0;  // 0: in order on the plan's stream; l > 0: on side stream l, concurrently within the step
 };
 
+// a captured hipGraph of `steps` steps of a plan
+struct hb_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int steps = 0;
+};
+
+#define HB_PLAN_GRAPHS 2  // graph lengths kept per plan (a run's whole graphs and its remainder)
+
 struct hb_plan {
     hb_ctx* ctx;
     int64_t* clock;
     std::vector<hb_launch_rec> launches;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int graph_steps = 0;
+    hb_graph graphs[HB_PLAN_GRAPHS];
+    int next_graph = 0;                                 // slot replaced by the next capture
+    bool timing = false;                                // record device time of hb_plan_run calls
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timers;
 };
 
+static int graph_drop(hb_graph& g) {
+    if (g.exec) HB_HIP(hipGraphExecDestroy(g.exec));
+    if (g.graph) HB_HIP(hipGraphDestroy(g.graph));
+    g.exec = nullptr;
+    g.graph = nullptr;
+    g.steps = 0;
+    return 0;
+}
+
 static int plan_drop_graph(hb_plan* p) {
-    if (p->exec) HB_HIP(hipGraphExecDestroy(p->exec));
-    if (p->graph) HB_HIP(hipGraphDestroy(p->graph));
-    p->exec = nullptr;
-    p->graph = nullptr;
-    p->graph_steps = 0;
+    for (auto& g : p->graphs) {
+        int rc = graph_drop(g);
+        if (rc) return rc;
+    }
     return 0;
 }
 
@@ -390,6 +408,32 @@ static int plan_enqueue(hb_plan* p, hipStream_t s, int64_t k) {
     return 0;
 }
 
+static int plan_capture(hb_plan* p, hipStream_t s, int steps, hipGraphExec_t* out) {
+    for (auto& g : p->graphs)
+        if (g.exec && g.steps == steps) {
+            *out = g.exec;
+            return 0;
+        }
+    hb_graph& g = p->graphs[p->next_graph];
+    p->next_graph = (p->next_graph + 1) % HB_PLAN_GRAPHS;
+    int rc = graph_drop(g);
+    if (rc) return rc;
+    HB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = plan_enqueue(p, s, steps);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &gr);
+    if (rc) {
+        if (gr) hipGraphDestroy(gr);
+        return rc;
+    }
+    if (e != hipSuccess) return set_err(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    g.graph = gr;
+    HB_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    g.steps = steps;
+    *out = g.exec;
+    return 0;
+}
+
 extern "C" int hb_plan_run(hb_plan* p, void* stream, int64_t n_steps, int graph_steps) {
     if (n_steps <= 0) return 0;
     hb_ctx* c = p->ctx;
@@ -398,32 +442,53 @@ extern "C" int hb_plan_run(hb_plan* p, void* stream, int64_t n_steps, int graph_
     // order after the caller's stream
     HB_HIP(hipEventRecord(c->ev_in, (hipStream_t)stream));
     HB_HIP(hipStreamWaitEvent(s, c->ev_in, 0));
+    std::pair<hipEvent_t, hipEvent_t> tm(nullptr, nullptr);
+    if (p->timing) {
+        HB_HIP(hipEventCreate(&tm.first));
+        HB_HIP(hipEventCreate(&tm.second));
+        p->timers.push_back(tm);
+        HB_HIP(hipEventRecord(tm.first, s));
+    }
     int64_t done = 0;
     if (graph_steps > 1 && n_steps >= graph_steps) {
-        if (!p->exec || p->graph_steps != graph_steps) {
-            int rc = plan_drop_graph(p);
-            if (rc) return rc;
-            HB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            rc = plan_enqueue(p, s, graph_steps);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(s, &g);
-            if (rc) return rc;
-            if (e != hipSuccess) return set_err(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-            p->graph = g;
-            HB_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
-            p->graph_steps = graph_steps;
-        }
+        hipGraphExec_t exec = nullptr;
+        int rc = plan_capture(p, s, graph_steps, &exec);
+        if (rc) return rc;
         int64_t reps = n_steps / graph_steps;
-        for (int64_t r = 0; r < reps; r++) HB_HIP(hipGraphLaunch(p->exec, s));
+        for (int64_t r = 0; r < reps; r++) HB_HIP(hipGraphLaunch(exec, s));
         done = reps * graph_steps;
     }
     if (n_steps > done) {
         int rc = plan_enqueue(p, s, n_steps - done);
         if (rc) return rc;
     }
+    if (p->timing) HB_HIP(hipEventRecord(tm.second, s));
     HB_HIP(hipEventRecord(c->ev_out, s));
     HB_HIP(hipStreamWaitEvent((hipStream_t)stream, c->ev_out, 0));
     return 0;
+}
+
+extern "C" int hb_plan_timing(hb_plan* p, int on, double* ms, int64_t* n_runs) {
+    double total = 0.0;
+    int64_t n = 0;
+    int rc = 0;
+    for (auto& tm : p->timers) {
+        if (!rc) {
+            hipError_t e = hipEventSynchronize(tm.second);
+            float x = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&x, tm.first, tm.second);
+            if (e != hipSuccess) rc = set_err(std::string("hb_plan_timing: ") + hipGetErrorString(e));
+            total += x;
+            n++;
+        }
+        hipEventDestroy(tm.first);
+        hipEventDestroy(tm.second);
+    }
+    p->timers.clear();
+    p->timing = on != 0;
+    if (ms) *ms = total;
+    if (n_runs) *n_runs = n;
+    return rc;
 }
 
 extern "C" int hb_plan_profile(hb_plan* p, void* stream, int64_t n_steps, float* ms_per_launch) {
@@ -480,6 +545,11 @@ extern "C" int hb_plan_profile(hb_plan* p, void* stream, int64_t n_steps, float*
 
 extern "C" int hb_plan_destroy(hb_plan* p) {
     if (!p) return 0;
+    for (auto& tm : p->timers) {
+        hipEventSynchronize(tm.second);
+        hipEventDestroy(tm.first);
+        hipEventDestroy(tm.second);
+    }
     int rc = plan_drop_graph(p);
     delete p;
     return rc;

@@ -1160,9 +1160,10 @@ class PostPathKernel:
 class SummedKernel:
     """A summed variable (synapses.py:1146-1200, SummedVariableUpdater; template
     summed_variable.pyx): target[j] = sum over the synapses of j, in synapse order,
-    of the expression, starting from 0.  One thread per target, the synapses of the
-    target (CSR by target, synapse order) folded sequentially, as the template's
-    loop over all synapses does for each target's subsequence."""
+    of the expression, starting from 0, for the targets j of the synapses' target
+    (sub)group.  One thread per target, the synapses of the target (CSR by target,
+    synapse order) folded sequentially, as the template's loop over all synapses
+    does for each target's subsequence."""
 
     BLOCK = 256
 
@@ -1173,8 +1174,8 @@ class SummedKernel:
         ps.ptr("clock", "long long", const=True)
         ps.val("step_off", "long long")
         ps.val("dt", "double")
-        ps.val("N", "int")
-        ps.val("pad", "int")
+        ps.val("t0", "int")  # the targets [t0, t1) of the group (a subgroup target: its range)
+        ps.val("t1", "int")
         ps.ptr("post_ptr", "unsigned int", const=True)
         ps.ptr("post_syn", "unsigned int", const=True)
         ps.ptr("pre_of", "int", const=True)
@@ -1201,8 +1202,8 @@ class SummedKernel:
             "    const long long step = p.clock[0] + p.step_off;",
             "    const double t = (double)step * p.dt;",
             "    (void)t;",
-            f"    const int j = blockIdx.x * {B} + threadIdx.x;",
-            "    if (j >= p.N) return;",
+            f"    const int j = p.t0 + (int)(blockIdx.x * {B} + threadIdx.x);",
+            "    if (j >= p.t1) return;",
             "    const int _postsynaptic_idx = j;",
             f"    {target_t} hb_acc = ({target_t})0;",
             "    for (unsigned int k_ = p.post_ptr[j]; k_ < p.post_ptr[j + 1]; k_++) {",

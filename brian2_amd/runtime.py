@@ -326,9 +326,12 @@ class Synapses:
             self.source, self.src_range = source.group, (source.start, source.stop)
         else:
             self.source, self.src_range = source, (0, source.N)
+        # a subgroup target is its group with an index range; connect()'s target indices are
+        # relative to it (_target_offset, synapses.py:1393-1396,2005-2006)
         if isinstance(target, Subgroup):
-            raise NotImplementedError("subgroup targets are not supported by the hip target yet")
-        self.target = target
+            self.target, self.tgt_range = target.group, (target.start, target.stop)
+        else:
+            self.target, self.tgt_range = target, (0, target.N)
         self.spec = model
         self.storage = dict(storage or {})
         self._analyse()
@@ -416,13 +419,18 @@ class Synapses:
         self.syn_vars = sorted(used | self.plastic)
 
     def connect(self, i, j):
+        """Synapses from source i to target j (indices relative to the source and target,
+        which may be subgroups; synapse index = position in the arrays)."""
         i = np.asarray(i, np.int64)
         j = np.asarray(j, np.int64)
         n_src = self.src_range[1] - self.src_range[0]
-        if len(i) and (i.min() < 0 or i.max() >= n_src or j.min() < 0 or j.max() >= self.target.N):
+        n_tgt = self.tgt_range[1] - self.tgt_range[0]
+        if len(i) != len(j):
+            raise ValueError("i and j must have the same length")
+        if len(i) and (i.min() < 0 or i.max() >= n_src or j.min() < 0 or j.max() >= n_tgt):
             raise IndexError("synapse indices out of range")
         self.pre = (i + self.src_range[0]).astype(np.int32)
-        self.post = j.astype(np.int32)
+        self.post = (j + self.tgt_range[0]).astype(np.int32)
 
     def set(self, name, values):
         if self.pre is None:
@@ -841,10 +849,16 @@ class Network:
         total = sum(int(s.csr["n_syn"]) for s in incoming)
         for s in incoming:
             self._gather_units(s, window)
+        # in-degree of every target over the incoming pathways (N_incoming, synapses.py:1401-1410):
+        # hb_csr_count with the targets as sources and one delay segment is a histogram
         indeg_n = torch.zeros(g.N, dtype=torch.int64, device="cuda")
+        cnt = zeros(g.N, "int")
         for s in incoming:
-            post = s.csr["post"][: s.csr["n_syn"]].long()
-            indeg_n += torch.bincount(post, minlength=g.N)
+            cnt.zero_()
+            _lib.check(L.hb_csr_count(ptr(s.csr["post"]), None, int(s.csr["n_syn"]), 0, g.N, 0, 1, ptr(cnt),
+                                      self.device.stream), "in-degree count")
+            indeg_n += cnt.long()
+        del cnt
         indeg_t = torch.nn.functional.pad(indeg_n, (0, g.n_tiles * T - g.N)).view(g.n_tiles, T).sum(1)
         indeg_t = indeg_t.cpu().numpy().astype(np.int64)
         need_tile = int(indeg_t.max()) if total else 0
@@ -1239,7 +1253,8 @@ class Network:
             K = SummedKernel(f"{s.name}_{tv}", sp.vector, used, pre_r, post_r, var_types, g.vars[tv], sc, g.float_t)
             fn = self.device.function(K.source, K.kernel_name)
             cls = K.struct.ctypes_struct()
-            v = {"clock": self.clock.data_ptr(), "step_off": 0, "dt": self.dt, "N": g.N,
+            t0, t1 = s.tgt_range
+            v = {"clock": self.clock.data_ptr(), "step_off": 0, "dt": self.dt, "t0": t0, "t1": t1,
                  "post_ptr": s.csr["post_ptr"].data_ptr(), "post_syn": s.csr["post_syn"].data_ptr(),
                  "pre_of": s.csr["pre_of"].data_ptr(), "target": g._dev[tv].data_ptr()}
             for n in used:
@@ -1255,7 +1270,7 @@ class Network:
                 if name in v:
                     setattr(a, name, v[name])
             s.summed_kernels.append(K)
-            self._add_launch(fn, (g.N + K.BLOCK - 1) // K.BLOCK, K.BLOCK, a, cls)
+            self._add_launch(fn, max(1, (t1 - t0 + K.BLOCK - 1) // K.BLOCK), K.BLOCK, a, cls)
 
     def _build_post(self, s):
         host = {"dt": self.dt}
@@ -1380,8 +1395,12 @@ class Network:
         self.t_step = snap["t_step"]
 
     # -- run ------------------------------------------------------------------
-    def run(self, n_steps=None, duration=None, graph_steps=None):
-        """Advances the network by n_steps time steps (or duration seconds)."""
+    def run(self, n_steps=None, duration=None, graph_steps=None, timing=False):
+        """Advances the network by n_steps time steps (or duration seconds).
+
+        timing: record HIP events on the launch stream around every enqueued chunk of
+        steps; `last_run_gpu_ms` is then the device time of the step launches (the end-of-
+        run fold excluded), without the host round trips between chunks."""
         torch = _torch()
         if n_steps is None:
             n_steps = int(round(duration / self.dt))
@@ -1406,19 +1425,27 @@ class Network:
                 m.buf = zeros(max(n_steps, 1), "int")
                 self._set_launch_args(g.launch_index, self._launch_args(g))
         done = 0
+        if timing:  # HIP events on the plan's stream around each chunk's launches
+            _lib.check(L.hb_plan_timing(self._plan, 1, None, None), "hb_plan_timing")
         while done < n_steps:
             chunk = n_steps - done
             for g in self.groups:
                 m = getattr(g, "smon", None)
                 if m is not None:
                     chunk = min(chunk, self._ensure_monitor_room(g, m, gs))
-            k = min(chunk, gs * max(1, chunk // gs)) if chunk >= gs else chunk
-            _lib.check(L.hb_plan_run(self._plan, stream, k, gs if k >= gs else 1), "hb_plan_run")
+            # whole graphs of gs steps, then the rest as one graph of its own length (a run
+            # shorter than gs steps is still replayed from a captured graph)
+            k = gs * (chunk // gs) if chunk >= gs else chunk
+            _lib.check(L.hb_plan_run(self._plan, stream, k, gs if k >= gs else k), "hb_plan_run")
             done += k
             self.t_step += k
             st = int(self.status.cpu().item())  # synchronises
             if st:
                 raise _lib.HipBrianError(self._status_message(st))
+        if timing:
+            ms, nr = ctypes.c_double(), ctypes.c_int64()
+            _lib.check(L.hb_plan_timing(self._plan, 0, ctypes.byref(ms), ctypes.byref(nr)), "hb_plan_timing")
+            self.last_run_gpu_ms = ms.value
         # fold the events delivered by the last step into the state (end of run)
         for g in self.groups:
             if getattr(g, "flush_fn", None) is not None:

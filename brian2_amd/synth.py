@@ -187,16 +187,19 @@ def cuba_network(N=4000, NE=None, p=0.02, seed=SEED, float_t="double", we=None, 
 
 
 def cuba_synw_network(N=5000, K=1000, dmin=1, dmax=50, seed=SEED, ginh=0.4, float_t="double", monitor=True,
-                      record=None, bucket_cap=None, graph_steps=100, network_options=None):
-    """CUBA with fixed in-degree K, per-synapse fp32 weights and delays (the 1M-neuron workload)."""
+                      record=None, bucket_cap=None, graph_steps=100, network_options=None, count_events=False):
+    """CUBA with fixed in-degree K, per-synapse fp32 weights and delays (the 1M-neuron workload).
+    count_events: per-synapse delivered-event counters on both pathways."""
     NE = N * 4 // 5
     m = Model.load("cuba_synw")
     G = NeuronGroup(N, m, float_t=float_t)
     G.v = cuba_v0(seed, N)
     we_base = 1.62e-3 * (80.0 / K)
     wi_base = -9e-3 * (80.0 / K) * ginh
-    Ce = Synapses(G[:NE], G, m.pathways["exc"], name="synapses", storage={"w": np.float32})
-    Ci = Synapses(G[NE:], G, m.pathways["inh"], name="synapses_1", storage={"w": np.float32})
+    Ce = Synapses(G[:NE], G, m.pathways["exc"], name="synapses", storage={"w": np.float32},
+                  count_events=count_events)
+    Ci = Synapses(G[NE:], G, m.pathways["inh"], name="synapses_1", storage={"w": np.float32},
+                  count_events=count_events)
     pop = None
     n_syn = 0
     for S, (a, b), wb in ((Ce, (0, NE), we_base), (Ci, (NE, N), wi_base)):

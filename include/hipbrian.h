@@ -104,9 +104,17 @@ int hb_plan_set_lane(hb_plan* plan, int index, int lane);
 /* Replaces the argument buffer of launch `index` (drops a captured graph). */
 int hb_plan_set_args(hb_plan* plan, int index, const void* args, size_t argsize);
 /* Runs n_steps steps, replaying a hipGraph of `graph_steps` steps when
- * graph_steps > 1, ordered after prior work on `stream`.  Returns after
- * enqueueing. */
+ * graph_steps > 1 (the remaining n_steps mod graph_steps steps are enqueued one
+ * launch at a time), ordered after prior work on `stream`.  A plan keeps the
+ * graphs of its two most recently used lengths.  Returns after enqueueing. */
 int hb_plan_run(hb_plan* plan, void* stream, int64_t n_steps, int graph_steps);
+/* Device time of hb_plan_run calls (Network.run's profile / get_profiling_info,
+ * core/network.py:461-486, at the granularity of a run call): returns in
+ * *ms the summed time between a HIP event recorded on the plan's stream before the
+ * first launch of each hb_plan_run call since the previous hb_plan_timing call and one
+ * after its last launch, and in *n_runs the number of such calls (waits for them);
+ * then turns the recording on (on != 0) or off for the calls that follow. */
+int hb_plan_timing(hb_plan* plan, int on, double* ms, int64_t* n_runs);
 /* Runs n_steps steps as individual launches with a HIP event after each one, on
  * the plan's stream, and accumulates each launch's duration over the steps into
  * ms_per_launch[index] (synchronous). */

@@ -21,12 +21,22 @@
  *  - PoissonGroup threshold (HBR_POISSON): the spike rate of every neuron within
  *    binomial bounds of rates * dt, and independence across neurons and steps
  *    (the reference draws with numpy, so only statistics can be compared).
+ *  - queues kept across runs with a dt change in between: the known answer of
+ *    brian2/tests/test_synapses.py:1310-1337, and the re-binning against the
+ *    reference's CSpikeQueue (oracle/_ref).
+ *  - the OpenMP variant (threads > 1, the CPU baseline's multi-core leg): final
+ *    state, spikes and event counts identical to one thread's.
  */
 #include "hbref.h"
 
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#else
+static int omp_get_thread_num(void) { return 0; }
+#endif
 
 #include "../brian2_amd/csrc/hb_math.h"
 
@@ -99,36 +109,62 @@ static int cmp_i32(const void* a, const void* b) {
 int hbr_fixed_indegree(uint64_t seed, uint64_t stream, int32_t n_pre, int32_t n_post, int32_t K,
                        int32_t* pre_of_post) {
     if (K > n_pre) return fail("fixed in-degree larger than the source population");
-    int32_t* buf = (int32_t*)malloc(sizeof(int32_t) * (size_t)K * 2);
-    for (int32_t j = 0; j < n_post; j++) {
-        int32_t have = 0;
-        int r = 0;
-        while (have < K) {
-            if (r >= 64) {
-                free(buf);
-                return fail("fixed in-degree sampling did not converge");
+    int failed = 0;
+    /* targets are independent draws: any number of threads gives the same output */
+#pragma omp parallel reduction(| : failed)
+    {
+        int32_t* buf = (int32_t*)malloc(sizeof(int32_t) * (size_t)K * 2);
+#pragma omp for schedule(dynamic, 256)
+        for (int32_t j = 0; j < n_post; j++) {
+            int32_t have = 0;
+            int r = 0;
+            while (have < K) {
+                if (r >= 64) {
+                    failed = 1;
+                    break;
+                }
+                int32_t m = K - have;
+                for (int32_t t = 0; t < m; t++) {
+                    uint64_t ctr = ((uint64_t)j * 64 + (uint64_t)r) * (uint64_t)K + (uint64_t)t;
+                    buf[have + t] = (int32_t)hbr_uniform_int(hbr_rand64(seed, stream, ctr), (uint32_t)n_pre);
+                }
+                qsort(buf, (size_t)K, sizeof(int32_t), cmp_i32);
+                int32_t u = 0;
+                for (int32_t t = 0; t < K; t++)
+                    if (t == 0 || buf[t] != buf[u - 1]) buf[u++] = buf[t];
+                have = u;
+                r++;
             }
-            int32_t m = K - have;
-            for (int32_t t = 0; t < m; t++) {
-                uint64_t ctr = ((uint64_t)j * 64 + (uint64_t)r) * (uint64_t)K + (uint64_t)t;
-                buf[have + t] = (int32_t)hbr_uniform_int(hbr_rand64(seed, stream, ctr), (uint32_t)n_pre);
-            }
-            qsort(buf, (size_t)K, sizeof(int32_t), cmp_i32);
-            int32_t u = 0;
-            for (int32_t t = 0; t < K; t++)
-                if (t == 0 || buf[t] != buf[u - 1]) buf[u++] = buf[t];
-            have = u;
-            r++;
+            memcpy(pre_of_post + (size_t)j * K, buf, sizeof(int32_t) * (size_t)K);
         }
-        memcpy(pre_of_post + (size_t)j * K, buf, sizeof(int32_t) * (size_t)K);
+        free(buf);
     }
-    free(buf);
-    return 0;
+    return failed ? fail("fixed in-degree sampling did not converge") : 0;
+}
+
+/* The synapses of a fixed in-degree table (n_post rows of K ascending sources) in
+ * (source, target) order, the order Synapses.connect(i=, j=) receives them in: a
+ * counting sort by source of the target-major table, stable, so each source's
+ * targets stay ascending. */
+void hbr_order_by_pre(const int32_t* pre_of_post, int32_t n_pre, int32_t n_post, int32_t K, int32_t* pre,
+                      int32_t* post) {
+    int64_t* start = (int64_t*)calloc((size_t)n_pre + 1, sizeof(int64_t));
+    const int64_t S = (int64_t)n_post * K;
+    for (int64_t e = 0; e < S; e++) start[pre_of_post[e] + 1]++;
+    for (int32_t i = 0; i < n_pre; i++) start[i + 1] += start[i];
+    for (int64_t e = 0; e < S; e++) {
+        int32_t i = pre_of_post[e];
+        int64_t q = start[i]++;
+        pre[q] = i;
+        post[q] = (int32_t)(e / K);
+    }
+    free(start);
 }
 
 /* lo + uniform_int(h(pair), hi - lo + 1) per synapse, keyed by (pre, post) */
 void hbr_pair_int(uint64_t seed, uint64_t stream, int32_t n_post, const int32_t* pre,
                   const int32_t* post, int64_t n, int32_t lo, int32_t hi, int32_t* out) {
+#pragma omp parallel for schedule(static)
     for (int64_t s = 0; s < n; s++) {
         uint64_t ctr = (uint64_t)pre[s] * (uint64_t)n_post + (uint64_t)post[s];
         out[s] = lo + (int32_t)hbr_uniform_int(hbr_rand64(seed, stream, ctr), (uint32_t)(hi - lo + 1));
@@ -138,6 +174,7 @@ void hbr_pair_int(uint64_t seed, uint64_t stream, int32_t n_post, const int32_t*
 /* (float)(lo + u (hi - lo)) per synapse, keyed by (pre, post) */
 void hbr_pair_uniform_f32(uint64_t seed, uint64_t stream, int32_t n_post, const int32_t* pre,
                           const int32_t* post, int64_t n, double lo, double hi, float* out) {
+#pragma omp parallel for schedule(static)
     for (int64_t s = 0; s < n; s++) {
         uint64_t ctr = (uint64_t)pre[s] * (uint64_t)n_post + (uint64_t)post[s];
         out[s] = (float)(lo + hbr_uniform(seed, stream, ctr) * (hi - lo));
@@ -346,9 +383,10 @@ static void cuba_scalar(const double* P, double dt, cuba_consts* c) {
     c->lio15 = hb_exp(c->lio7);
 }
 
-static void cuba_update(const cuba_consts* c, hbr_group* g, double t, double dt) {
+static void cuba_update(const cuba_consts* c, hbr_group* g, double t, double dt, int threads) {
     int32_t N = g->N;
     double *v = g->state, *ge = g->state + N, *gi = g->state + 2 * (size_t)N;
+#pragma omp parallel for num_threads(threads) schedule(static) if (threads > 1)
     for (int32_t i = 0; i < N; i++) {
         int nr = timestep(t - g->lastspike[i], dt) >= c->lio1;
         g->not_refractory[i] = (uint8_t)nr;
@@ -382,9 +420,10 @@ static void lif_scalar(const double* P, double dt, lif_consts* c) {
     c->lio4 = hb_exp((-dt) / tau_m);
 }
 
-static void lif_update(const lif_consts* c, hbr_group* g) {
+static void lif_update(const lif_consts* c, hbr_group* g, int threads) {
     int32_t N = g->N;
     double *v = g->state, *I = g->state + N;
+#pragma omp parallel for num_threads(threads) schedule(static) if (threads > 1)
     for (int32_t i = 0; i < N; i++) {
         double _I = c->lio1 * I[i];
         double _v = c->lio2 + ((c->lio3 * I[i]) + (c->lio4 * v[i]));
@@ -437,12 +476,13 @@ static void hh_scalar(const double* P, double dt, hh_consts* c) {
     c->thr = (-20.0) * mV; /* threshold block: _lio_1 := (-20.0) * mV */
 }
 
-static void hh_update(const hh_consts* c, hbr_group* g, double t, double dt) {
+static void hh_update(const hh_consts* c, hbr_group* g, double t, double dt, int threads) {
     int32_t N = g->N;
     const double* l = c->l;
     double ms = c->ms;
     double *V = g->state, *M = V + N, *Nn = V + 2 * (size_t)N, *H = V + 3 * (size_t)N;
     double *GE = V + 4 * (size_t)N, *GI = V + 5 * (size_t)N;
+#pragma omp parallel for num_threads(threads) schedule(static) if (threads > 1)
     for (int32_t i = 0; i < N; i++) {
         double v = V[i], m = M[i], n = Nn[i], h = H[i], ge = GE[i], gi = GI[i];
         g->not_refractory[i] = (uint8_t)(timestep(t - g->lastspike[i], dt) >= c->lio1);
@@ -483,6 +523,74 @@ static void hh_update(const hh_consts* c, hbr_group* g, double t, double dt) {
 /* network.py:42-50): groups -> thresholds (+ monitor, order+1) -> synapses   */
 /* (push, then propagate, per pathway) -> resets -> clock.advance.             */
 /* ------------------------------------------------------------------------- */
+/* The left fold of one slot split by target over T threads (the OpenMP leg of the CPU
+ * baseline, not a reference restatement): thread t owns the targets [t*N/T, (t+1)*N/T).
+ * The slot is cut into T chunks; each thread counts its chunk's events per owner, a
+ * scan gives every (chunk, owner) pair its place, each thread copies its chunk's events
+ * to the owners' lists in slot order, and each owner then folds its list in order.  A
+ * target's events are thus applied in queue order, as in the sequential loop. */
+typedef struct {
+    int32_t *syn, *post;
+    int64_t cap;
+    int64_t* cnt;   /* [T][T] */
+    int32_t T;
+} fold_buf;
+
+static void fold_parallel(const hbr_pathway* P, double* tv, int32_t N, const int32_t* slot, int64_t n, int T,
+                          fold_buf* fb) {
+    if (fb->cap < n) {
+        free(fb->syn);
+        free(fb->post);
+        fb->cap = n * 2;
+        fb->syn = (int32_t*)malloc(sizeof(int32_t) * (size_t)fb->cap);
+        fb->post = (int32_t*)malloc(sizeof(int32_t) * (size_t)fb->cap);
+    }
+    if (fb->T != T) {
+        free(fb->cnt);
+        fb->cnt = (int64_t*)malloc(sizeof(int64_t) * (size_t)(T * T + 1));
+        fb->T = T;
+    }
+    int64_t* cnt = fb->cnt;
+#pragma omp parallel num_threads(T)
+    {
+        int t = omp_get_thread_num();
+        int64_t a = n * t / T, b = n * (t + 1) / T;
+        int64_t* c = cnt + (size_t)t * T;
+        for (int o = 0; o < T; o++) c[o] = 0;
+        for (int64_t e = a; e < b; e++) c[(int64_t)P->post[slot[e]] * T / N]++;
+#pragma omp barrier
+#pragma omp single
+        {
+            /* owner-major positions: owner o's list holds chunk 0's events, then chunk 1's ... */
+            int64_t acc = 0;
+            for (int o = 0; o < T; o++)
+                for (int k = 0; k < T; k++) {
+                    int64_t x = cnt[(size_t)k * T + o];
+                    cnt[(size_t)k * T + o] = acc;
+                    acc += x;
+                }
+        }
+        for (int64_t e = a; e < b; e++) {
+            int32_t syn = slot[e], post = P->post[syn];
+            int64_t q = c[(int64_t)post * T / N]++;
+            fb->syn[q] = syn;
+            fb->post[q] = post;
+        }
+#pragma omp barrier
+        /* after the copy cnt[k][o] is the end of chunk k's part of owner o's list, so owner
+           t's list ends at cnt[T-1][t] and starts where owner t-1's ends */
+        const int64_t lo = t > 0 ? cnt[(size_t)(T - 1) * T + t - 1] : 0;
+        const int64_t hi = cnt[(size_t)(T - 1) * T + t];
+        for (int64_t q = lo; q < hi; q++) {
+            int32_t syn = fb->syn[q], post = fb->post[q];
+            double x = tv[post];
+            x += P->w ? P->w[syn] : P->w_scalar;
+            tv[post] = x;
+            if (P->counts) P->counts[syn]++;
+        }
+    }
+}
+
 typedef struct {
     cuba_consts cuba;
     lif_consts lif;
@@ -507,10 +615,18 @@ int hbr_run(hbr_sim* S) {
         }
     }
     /* SynapticPathway.initialise_queue -> CSpikeQueue::prepare (synapses.py:421-458) */
-    hbr_queue** qs = (hbr_queue**)calloc((size_t)(S->n_paths > 0 ? S->n_paths : 1), sizeof(hbr_queue*));
+    /* the queues: the caller's (kept across runs, prepared again with this run's dt,
+       synapses.py:421-437) or this call's own */
+#ifdef _OPENMP
+    const int threads = S->threads > 1 ? S->threads : 1;
+#else
+    const int threads = 1;
+#endif
+    hbr_queue** qs = S->queues ? S->queues
+                               : (hbr_queue**)calloc((size_t)(S->n_paths > 0 ? S->n_paths : 1), sizeof(hbr_queue*));
     for (int32_t p = 0; p < S->n_paths; p++) {
         hbr_pathway* P = &S->paths[p];
-        qs[p] = hbr_queue_create(P->source_start, P->source_end);
+        if (!qs[p]) qs[p] = hbr_queue_create(P->source_start, P->source_end);
         int rc;
         if (P->delay)
             rc = hbr_queue_prepare(qs[p], P->delay, P->n_syn, P->pre, P->n_syn, dt);
@@ -518,6 +634,7 @@ int hbr_run(hbr_sim* S) {
             rc = hbr_queue_prepare(qs[p], &P->delay_scalar, 1, P->pre, P->n_syn, dt);
         if (rc) return rc;
     }
+    fold_buf fb = {0};
     int32_t* slot = NULL;
     int64_t slot_cap = 0;
     S->n_spk = 0;
@@ -536,9 +653,9 @@ int hbr_run(hbr_sim* S) {
         /* 'groups': state updaters */
         for (int32_t gi = 0; gi < S->n_groups; gi++) {
             hbr_group* g = &S->groups[gi];
-            if (g->model == HBR_CUBA_EXACT) cuba_update(&rt[gi].cuba, g, t, dt);
-            else if (g->model == HBR_COBAHH_EE) hh_update(&rt[gi].hh, g, t, dt);
-            else if (g->model == HBR_LIF_EXACT) lif_update(&rt[gi].lif, g);
+            if (g->model == HBR_CUBA_EXACT) cuba_update(&rt[gi].cuba, g, t, dt, threads);
+            else if (g->model == HBR_COBAHH_EE) hh_update(&rt[gi].hh, g, t, dt, threads);
+            else if (g->model == HBR_LIF_EXACT) lif_update(&rt[gi].lif, g, threads);
         }
         /* 'thresholds': threshold.pyx:18-32 / spikegenerator.pyx:6-26 */
         for (int32_t gi = 0; gi < S->n_groups; gi++) {
@@ -603,14 +720,18 @@ int hbr_run(hbr_sim* S) {
             hbr_queue_peek(qs[p], slot, slot_cap);
             hbr_group* tg = &S->groups[P->target_group];
             double* tv = tg->state + (size_t)P->target_var * tg->N;
-            /* synapses.pyx:42-50: sequential left fold in queue order */
-            for (int64_t e = 0; e < n; e++) {
-                int32_t syn = slot[e];
-                int32_t post = P->post[syn];
-                double x = tv[post];
-                x += P->w ? P->w[syn] : P->w_scalar;
-                tv[post] = x;
-                if (P->counts) P->counts[syn]++;
+            if (threads > 1 && n >= 4096) {
+                fold_parallel(P, tv, tg->N, slot, n, threads, &fb);
+            } else {
+                /* synapses.pyx:42-50: sequential left fold in queue order */
+                for (int64_t e = 0; e < n; e++) {
+                    int32_t syn = slot[e];
+                    int32_t post = P->post[syn];
+                    double x = tv[post];
+                    x += P->w ? P->w[syn] : P->w_scalar;
+                    tv[post] = x;
+                    if (P->counts) P->counts[syn]++;
+                }
             }
             hbr_queue_advance(qs[p]);
         }
@@ -626,9 +747,14 @@ int hbr_run(hbr_sim* S) {
             }
         }
     }
-    for (int32_t p = 0; p < S->n_paths; p++) hbr_queue_destroy(qs[p]);
+    if (!S->queues) {
+        for (int32_t p = 0; p < S->n_paths; p++) hbr_queue_destroy(qs[p]);
+        free(qs);
+    }
     for (int32_t gi = 0; gi < S->n_groups; gi++) free(rt[gi].spikes);
-    free(qs);
+    free(fb.syn);
+    free(fb.post);
+    free(fb.cnt);
     free(rt);
     free(slot);
     return rc;

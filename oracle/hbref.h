@@ -24,6 +24,8 @@ int64_t hbr_bernoulli(uint64_t seed, uint64_t stream, int32_t i0, int32_t i1, in
                       double p, int32_t* pre, int32_t* post, int64_t cap);
 int hbr_fixed_indegree(uint64_t seed, uint64_t stream, int32_t n_pre, int32_t n_post, int32_t K,
                        int32_t* pre_of_post);
+void hbr_order_by_pre(const int32_t* pre_of_post, int32_t n_pre, int32_t n_post, int32_t K, int32_t* pre,
+                      int32_t* post);
 void hbr_pair_int(uint64_t seed, uint64_t stream, int32_t n_post, const int32_t* pre,
                   const int32_t* post, int64_t n, int32_t lo, int32_t hi, int32_t* out);
 void hbr_pair_uniform_f32(uint64_t seed, uint64_t stream, int32_t n_post, const int32_t* pre,
@@ -94,6 +96,16 @@ typedef struct {
     const int32_t* rec_idx;
     const int32_t* rec_vars;
     double* rec_out;        /* [n_steps][n_rec_vars][n_rec] */
+    /* threads > 1: the neuron updates and each pathway's fold run on that many OpenMP
+       threads (the fold split by target, each target's events still applied in queue
+       order: the results are identical to one thread's) */
+    int32_t threads;
+    /* queues: NULL, or n_paths queue handles kept across calls the way a pathway keeps
+       its SpikeQueue across runs (synapses.py:421-437): a NULL entry is created on the
+       first call; every call prepares each queue with this call's dt, which re-bins the
+       spikes in flight when dt changed (spikequeue.h:58-74).  The caller destroys them
+       with hbr_queue_destroy. */
+    hbr_queue** queues;
 } hbr_sim;
 
 int hbr_run(hbr_sim* sim);

@@ -2,6 +2,7 @@
 
 Test infrastructure only: the oracle is the checker, never the thing measured.
 """
+import ast
 import ctypes
 import os
 import subprocess
@@ -42,6 +43,7 @@ class Sim(ctypes.Structure):
         ("n_paths", i32), ("paths", P(Pathway)), ("mon_group", i32), ("spk_i", P(i32)),
         ("spk_step", P(i64)), ("spk_cap", i64), ("n_spk", i64), ("rec_group", i32), ("n_rec", i32),
         ("n_rec_vars", i32), ("rec_idx", P(i32)), ("rec_vars", P(i32)), ("rec_out", P(dbl)),
+        ("threads", i32), ("queues", P(ctypes.c_void_p)),
     ]
 
 
@@ -66,6 +68,7 @@ def lib():
         L.hbr_fixed_indegree.restype = ctypes.c_int
         L.hbr_fixed_indegree.argtypes = [u64, u64, i32, i32, i32, P(i32)]
         L.hbr_pair_int.argtypes = [u64, u64, i32, P(i32), P(i32), i64, i32, i32, P(i32)]
+        L.hbr_order_by_pre.argtypes = [P(i32), i32, i32, i32, P(i32), P(i32)]
         L.hbr_pair_uniform_f32.argtypes = [u64, u64, i32, P(i32), P(i32), i64, dbl, dbl, P(ctypes.c_float)]
         L.hbr_queue_create.restype = ctypes.c_void_p
         L.hbr_queue_create.argtypes = [i32, i32]
@@ -224,9 +227,32 @@ class OraclePathway:
         self.counts = np.zeros(len(self.pre), np.int32) if count else None
 
 
+def cpu_threads():
+    """Threads for the oracle's OpenMP leg: OMP_NUM_THREADS when set, else the cores
+    this process may run on."""
+    n = os.environ.get("OMP_NUM_THREADS")
+    return int(n) if n else len(os.sched_getaffinity(0))
+
+
+class OracleQueues:
+    """The pathways' spike queues kept across oracle runs (a Synapses object keeps its
+    SpikeQueue from one run to the next, brian2/synapses/synapses.py:421-437)."""
+
+    def __init__(self, n_paths):
+        self.arr = (ctypes.c_void_p * max(1, n_paths))()
+
+    def __del__(self):
+        L = lib()
+        for q in self.arr:
+            if q:
+                L.hbr_queue_destroy(q)
+
+
 def run(groups, paths, dt, n_steps, start_step=0, mon_group=0, rec_group=0, rec_idx=(), rec_vars=(),
-        spk_cap=None):
-    """Runs the oracle; returns (spike_i, spike_step, rec [steps, vars, idx])."""
+        spk_cap=None, threads=1, queues=None):
+    """Runs the oracle; returns (spike_i, spike_step, rec [steps, vars, idx]).
+    queues: an OracleQueues kept across calls (spikes in flight survive the end of a
+    run; each call prepares the queues with its dt), else fresh queues per call."""
     L = lib()
     gs = (Group * len(groups))(*[g.struct() for g in groups])
     ps = (Pathway * max(len(paths), 1))()
@@ -259,6 +285,8 @@ def run(groups, paths, dt, n_steps, start_step=0, mon_group=0, rec_group=0, rec_
     sim.rec_group, sim.n_rec, sim.n_rec_vars = rec_group, len(rec_idx), len(rec_vars)
     sim.rec_idx, sim.rec_vars = ptr(rec_idx, i32), ptr(rec_vars, i32)
     sim.rec_out = ptr(rec_out, dbl)
+    sim.threads = int(threads)
+    sim.queues = ctypes.cast(queues.arr, P(ctypes.c_void_p)) if queues is not None else None
     rc = L.hbr_run(ctypes.byref(sim))
     if rc != 0:
         raise RuntimeError(L.hbr_last_error().decode())
@@ -327,12 +355,20 @@ def build_cobahh(meta):
     return G, [Ci, Ce]
 
 
+def order_by_pre(pre_of_post):
+    """Synapses of a fixed in-degree table in (source, target) order."""
+    n_post, K = pre_of_post.shape
+    n_pre = int(pre_of_post.max()) + 1 if pre_of_post.size else 0
+    pre = np.empty(n_post * K, np.int32)
+    post = np.empty(n_post * K, np.int32)
+    tab = np.ascontiguousarray(pre_of_post, np.int32)
+    lib().hbr_order_by_pre(ptr(tab, i32), n_pre, n_post, K, ptr(pre, i32), ptr(post, i32))
+    return pre, post
+
+
 def cuba_delay_synapses(meta):
     N, K, seed = meta["N"], meta["K"], meta["seed"]
-    pre = fixed_indegree(seed, N, N, K).ravel()
-    post = np.repeat(np.arange(N, dtype=np.int32), K)
-    order = np.lexsort((post, pre))
-    pre, post = pre[order], post[order]
+    pre, post = order_by_pre(fixed_indegree(seed, N, N, K))
     d = pair_int(seed, S_DELAY, N, pre, post, meta["dmin"], meta["dmax"])
     w = pair_uniform_f32(seed, S_W, N, pre, post, 0.5, 1.5)
     exc = pre < meta["NE"]
@@ -397,7 +433,7 @@ def build_lif(inp, trial, c=LIF):
 
 def load_golden(name):
     z = np.load(os.path.join(REPO, "tests", "golden", f"golden_{name}.npz"))
-    meta = eval(str(z["meta"]), {"__builtins__": {}}, {"True": True, "False": False})
+    meta = ast.literal_eval(str(z["meta"]))  # the fixture's metadata is a dict literal
     return z, meta
 
 

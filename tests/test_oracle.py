@@ -424,3 +424,41 @@ def test_poisson_oracle_rates_and_independence():
     i3, s3 = O.run([g2], [], dt, 500, start_step=7000, mon_group=0)[:2]
     sel = (s >= 7000) & (s < 7500)
     assert np.array_equal(i3, i[sel]) and np.array_equal(s3, s[sel])
+
+
+def test_oracle_openmp_equals_one_thread():
+    """The OpenMP leg of the oracle (the CPU baseline's multi-core figure) folds each
+    target's events in queue order: state, spikes and per-synapse counts are
+    identical to one thread's."""
+    z, meta = O.load_golden("cuba_delay")
+    out = []
+    for threads in (1, 4):
+        G, paths = O.build_cuba_delay(meta)
+        for p in paths:
+            p.counts = np.zeros(len(p.pre), np.int32)
+        i, s, _ = O.run([G], paths, meta["dt"], 1500, threads=threads)
+        out.append((i, s, G.state.copy(), [p.counts.copy() for p in paths]))
+    (i1, s1, st1, c1), (i4, s4, st4, c4) = out
+    assert np.array_equal(i1, z["i"][z["step"] < 1500]) and np.array_equal(i1, i4) and np.array_equal(s1, s4)
+    assert np.array_equal(st1, st4)
+    assert all(np.array_equal(a, b) for a, b in zip(c1, c4))
+    assert sum(int(c.sum()) for c in c1) > 100_000
+
+
+def test_oracle_queues_kept_across_runs():
+    """With the queues kept across runs (a pathway keeps its SpikeQueue), runs in
+    pieces equal one run, spikes in flight included; fresh queues per run lose them."""
+    z, meta = O.load_golden("cuba_delay")
+    G1, p1 = O.build_cuba_delay(meta)
+    O.run([G1], p1, meta["dt"], 600)
+    G2, p2 = O.build_cuba_delay(meta)
+    q = O.OracleQueues(len(p2))
+    a = O.run([G2], p2, meta["dt"], 250, queues=q)
+    b = O.run([G2], p2, meta["dt"], 350, start_step=250, queues=q)
+    assert np.array_equal(G1.state, G2.state)
+    i = np.concatenate([a[0], b[0]])
+    assert np.array_equal(i, z["i"][z["step"] < 600])
+    G3, p3 = O.build_cuba_delay(meta)
+    O.run([G3], p3, meta["dt"], 250)
+    O.run([G3], p3, meta["dt"], 350, start_step=250)
+    assert not np.array_equal(G1.state, G3.state)
