@@ -91,6 +91,32 @@ def cpu_baseline(n=20000, K=1000, warm=1000, steps=2000):
                        f"{out[1][1]} in {out[1][2]:.1f} s on one; final state identical: {same}")
 
 
+class RoctxRegion:
+    """roctxProfilerResume / roctxProfilerPause around the timed region, so that a
+    `rocprofv3 --selected-regions` run traces exactly the launches the bench
+    times (outside a profiler these calls do nothing)."""
+
+    def __init__(self, on):
+        self.lib = None
+        if on:
+            import ctypes
+
+            self.lib = ctypes.CDLL("/opt/rocm/lib/librocprofiler-sdk-roctx.so.1")
+            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
+            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
+            self.lib.roctxProfilerPause(0)
+
+    def __enter__(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+        return self
+
+    def __exit__(self, *exc):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+        return False
+
+
 def kernel_bytes(kind, n, events, spikes, fbytes):
     """Algorithmic bytes of one launch: neuron state read + write (v, ge, gi),
     lastspike read, not_refractory read + write; per spike the ring entry, the
@@ -193,42 +219,44 @@ def adjoint_bench(trials=256):
             "backward_ms": t_b * 1e3, "ckpt_every": net.C, "loss": loss, "dtype": "f64 state, f32 weights"}
 
 
-def fp32_error_pair(make, n_steps, chunk, variables):
-    """Runs one configuration in fp64 and in fp32 side by side, chunk by chunk.
-    Returns the spike counts of both, the first step at which the spike lists
-    differ (None: never), and per state variable the largest |x32 - x64| at a
-    chunk boundary before that step, absolute and relative to max |x64|."""
+def fp32_error_pair(make, n_steps, chunk, variables, fine_steps=2000):
+    """Runs one configuration in fp64 and in fp32 side by side.  Step by step
+    until the spike lists first differ (at most fine_steps steps), comparing
+    every state variable after each step; then in chunks to the end.  Returns
+    the spike counts of both, the first step at which the spike lists differ
+    (None: never), and per state variable the largest |x32 - x64| over the steps
+    before that one, absolute and relative to max |x64|."""
     import numpy as np
 
     c64, c32 = make("double"), make("float")
     err = {v: [0.0, 0.0] for v in variables}
     first_div = None
     done = 0
+    compared = 0
     while done < n_steps:
-        k = min(chunk, n_steps - done)
+        fine = first_div is None and done < fine_steps
+        k = 1 if fine else min(chunk, n_steps - done)
         c64["net"].run(k)
         c32["net"].run(k)
         done += k
-        if first_div is None:
-            i64, s64 = c64["mon"].i, c64["mon"].step
-            i32, s32 = c32["mon"].i, c32["mon"].step
-            if not (np.array_equal(i64, i32) and np.array_equal(s64, s32)):
-                m = min(len(s64), len(s32))
-                bad = np.nonzero((i64[:m] != i32[:m]) | (s64[:m] != s32[:m]))[0]
-                first_div = int(min(s64[bad[0]], s32[bad[0]]) if len(bad) else
-                                (s64[m] if len(s64) > m else s32[m]))
-                continue
-            for v in variables:
-                a, b = c64["G"].get(v), c32["G"].get(v).astype(np.float64)
-                scale = float(np.max(np.abs(a))) or 1.0
-                e = float(np.max(np.abs(a - b)))
-                err[v][0] = max(err[v][0], e)
-                err[v][1] = max(err[v][1], e / scale)
+        if not fine:
+            continue
+        n64, n32 = int(c64["mon"].mon_n.item()), int(c32["mon"].mon_n.item())
+        if n64 != n32 or (n64 and not (np.array_equal(c64["mon"].i, c32["mon"].i))):
+            first_div = done - 1
+            continue
+        compared += 1
+        for v in variables:
+            a, b = c64["G"].get(v), c32["G"].get(v).astype(np.float64)
+            scale = float(np.max(np.abs(a))) or 1.0
+            e = float(np.max(np.abs(a - b)))
+            err[v][0] = max(err[v][0], e)
+            err[v][1] = max(err[v][1], e / scale)
     n64, n32 = len(c64["mon"].i), len(c32["mon"].i)
     del c64, c32
     return {"steps": n_steps, "spikes_f64": n64, "spikes_f32": n32,
             "spike_count_rel_delta": (n32 - n64) / max(1, n64),
-            "first_divergent_step": first_div, "compared_every": chunk,
+            "first_divergent_step": first_div, "steps_compared_before_divergence": compared,
             "max_abs_err_before_divergence": {v: err[v][0] for v in variables},
             "max_rel_err_before_divergence": {v: err[v][1] for v in variables}}
 
@@ -295,6 +323,76 @@ def measured_traffic(kernel, sha1, workload):
     return {k: e[k] for k in ("fetch_bytes", "write_bytes", "bytes", "dispatches", "kernel_sha1") if k in e}
 
 
+def _rocprof_rows(d, suffix):
+    import csv
+    import glob
+
+    rows = []
+    for fn in sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)):
+        with open(fn, newline="") as f:
+            rows += list(csv.DictReader(f))
+    if not rows:
+        raise SystemExit(f"no *{suffix} under {d}")
+    return rows
+
+
+def summarize_trace(d):
+    """Markdown table of a `rocprofv3 --kernel-trace --output-format csv` run: per
+    kernel the dispatches, total / average / median / min / max duration, and for
+    the kernel with the most GPU time the idle gaps between its launches."""
+    import numpy as np
+
+    rows = _rocprof_rows(d, "kernel_trace.csv")
+    by = {}
+    for r in rows:
+        by.setdefault(r["Kernel_Name"], []).append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    total = sum(e - s for v in by.values() for s, e in v)
+    lines = ["| kernel | calls | total ms | avg us | median us | min us | max us | % of GPU time |",
+             "|---|---|---|---|---|---|---|---|"]
+    order = sorted(by, key=lambda k: -sum(e - s for s, e in by[k]))
+    for k in order:
+        dur = np.array([e - s for s, e in by[k]], np.float64) / 1e3
+        name = k if len(k) <= 60 else k[:57] + "..."
+        lines.append(f"| `{name}` | {len(dur)} | {dur.sum() / 1e3:.3f} | {dur.mean():.2f} | {np.median(dur):.2f} | "
+                     f"{dur.min():.2f} | {dur.max():.2f} | {dur.sum() / 1e3 / (total / 1e6) * 100:.1f} |")
+    top = sorted(by[order[0]])
+    st = np.array([s for s, _ in top], np.int64)
+    en = np.array([e for _, e in top], np.int64)
+    if len(top) > 1:
+        gap = (st[1:] - en[:-1]) / 1e3
+        lines.append("")
+        lines.append(f"`{order[0][:60]}`: start-to-start {np.mean(np.diff(st)) / 1e3:.2f} us on average; idle between "
+                     f"launches {gap.mean():.2f} us on average, {np.median(gap):.2f} us median; first start to last "
+                     f"end {(en[-1] - st[0]) / 1e6:.3f} ms")
+    return "\n".join(lines)
+
+
+def summarize_counters(d, counter, copy_bytes=256 << 20):
+    """Per-dispatch average of one counter (`rocprofv3 --pmc <counter> --output-format
+    csv`) per kernel, in KiB and in bytes scaled by what the same counter reads
+    for a device copy of copy_bytes (the copy kernel's dispatches in the run)."""
+    import numpy as np
+
+    rows = _rocprof_rows(d, "counter_collection.csv")
+    by = {}
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        by.setdefault(r["Kernel_Name"], {}).setdefault(r.get("Dispatch_Id") or r.get("Correlation_Id"), 0.0)
+        by[r["Kernel_Name"]][r.get("Dispatch_Id") or r.get("Correlation_Id")] += float(r["Counter_Value"])
+    copy = [k for k in by if "copy" in k.lower()]
+    if not copy:
+        raise SystemExit("no copy kernel in the counter run (run the bench with --copy-calibration)")
+    copy_kib = float(np.mean(list(by[copy[0]].values())))
+    factor = copy_bytes / 1024.0 / copy_kib
+    out = {"_calibration": {"counter": counter, "copy_kernel": copy[0], "copy_bytes": copy_bytes,
+                            "copy_kib_per_dispatch": copy_kib, "factor": factor}}
+    for k, v in by.items():
+        kib = float(np.mean(list(v.values())))
+        out[k] = {"dispatches": len(v), "kib_per_dispatch": kib, "bytes_per_dispatch": kib * 1024.0 * factor}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,7 +413,19 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with device copies of known size (for counter runs)")
     ap.add_argument("--sweep-steps", type=int, default=10000)
+    ap.add_argument("--roctx-region", action="store_true",
+                    help="pause profiling outside the timed region (for rocprofv3 --selected-regions)")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print a markdown summary of a rocprofv3 kernel trace (csv) and exit")
+    ap.add_argument("--summarize-counters", nargs=2, metavar=("COUNTER", "DIR"),
+                    help="print per-dispatch averages of a rocprofv3 counter run (csv) as JSON and exit")
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace))
+        return
+    if args.summarize_counters:
+        print(json.dumps(summarize_counters(args.summarize_counters[1], args.summarize_counters[0]), indent=1))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: there is no multi-GPU path (a network would shard by "
               "neuron range, exchanging each step's spike lists)", file=sys.stderr)
@@ -327,8 +437,10 @@ def main():
 
     SPS = args.sim_steps_per_step
     log(f"device: {torch.cuda.get_device_name(0)}")
+    region = RoctxRegion(args.roctx_region)
     if args.copy_calibration:
-        copy_calibration()
+        with region:
+            copy_calibration()
     t0 = time.perf_counter()
     c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=SPS,
                                 network_options={"fuse": not args.no_fuse})
@@ -348,10 +460,11 @@ def main():
     ev0 = net.n_events
     spk0 = int(mon.mon_n.item())
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    net.run(n_sim, timing=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    with region:
+        t0 = time.perf_counter()
+        net.run(n_sim, timing=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     gpu_ms = net.last_run_gpu_ms
     events = net.n_events - ev0
     spikes = int(mon.mon_n.item()) - spk0

@@ -1477,6 +1477,9 @@ class Network:
 
     @property
     def n_events(self):
+        """Synaptic events delivered since the network was built (0 before its first run)."""
+        if not self._built:
+            return 0
         return sum(s.n_events for s in self.paths)
 
     def profile(self, n_steps):
