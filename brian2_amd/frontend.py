@@ -100,10 +100,12 @@ def from_specs(desc, arrays, code_objects, float_t="double", **network_options):
         post_spec = CodeSpec(cos[pw["post"]["code_object"]]) if "post" in pw else None
         post_delay = 0.0
         if "post" in pw:
-            d = np.asarray(_arr(arrays, pw["post"]["delay"], name), np.float64)
-            if d.size != 1:
+            d = np.asarray(_arr(arrays, pw["post"]["delay"], name), np.float64).reshape(-1)
+            # Brian keeps a per-synapse delay array unless a scalar delay was given
+            # (synapses.py:258-313); the on_post pathway here takes one delay for all synapses
+            if d.size > 1 and np.any(d != d[0]):
                 raise NotImplementedError(f"{name}: per-synapse delays of an on_post pathway")
-            post_delay = float(d.reshape(-1)[0])
+            post_delay = float(d[0]) if d.size else 0.0
         summed = {x["target_var"]: CodeSpec(cos[x["code_object"]]) for x in s.get("summed", [])}
         storage = {k: np.float32 for k in s.get("float32_storage", [])}
         S = Synapses(src, tgt, pre_spec, name=name, on_post=post_spec, post_delay=post_delay, summed=summed,
@@ -164,23 +166,41 @@ def export_state(net):
             i, step = M.i, M.step
             # spikemonitor.pyx records the index relative to the monitored (sub)group
             out[f"_array_{name}_i"] = (i - M.range[0]).astype(np.int32)
-            out[f"_array_{name}_t"] = step.astype(np.float64) * dt
+            out[f"_array_{name}_t"] = M.t_
             out[f"_array_{name}_count"] = M.count.astype(np.int32) if M.count_dev is not None else \
                 np.zeros(M.range[1] - M.range[0], np.int32)
             out[f"_array_{name}_N"] = np.array([len(i)], np.int32)
         elif isinstance(M, StateMonitor):
             vals = M.values()  # [vars, record, steps]
             steps = vals.shape[2]
-            first = net.t_step - steps
-            out[f"_array_{name}_t"] = np.arange(first, first + steps, dtype=np.float64) * dt
+            out[f"_array_{name}_t"] = M.t_
             for k, v in enumerate(M.variables):
                 out[f"_array_{name}_{v}"] = vals[k].T.copy()
             out[f"_array_{name}_N"] = np.array([steps], np.int32)
         elif isinstance(M, PopulationRateMonitor):
-            out[f"_array_{name}_t"] = M.step.astype(np.float64) * dt
-            out[f"_array_{name}_rate"] = M.rate(dt)
+            out[f"_array_{name}_t"] = M.t_
+            out[f"_array_{name}_rate"] = M.rate()
             out[f"_array_{name}_N"] = np.array([len(M.step)], np.int32)
     clk = b["clock"]
     out[f"_array_{clk}_timestep"] = np.array([net.t_step], np.int64)
     out[f"_array_{clk}_t"] = np.array([net.t_step * dt], np.float64)
     return out
+
+
+def update_state(net, desc, arrays):
+    """Before a further run of a device network built by from_specs: uploads the
+    state that the caller may have changed between runs (every neuron group
+    variable and every synaptic variable, by Brian's array names)."""
+    b = net.brian
+    for g in desc["groups"]:
+        G = b["groups"][g["name"]]
+        if isinstance(G, NeuronGroup):
+            for v in G.vars:
+                an = f"_array_{g['name']}_{v}"
+                if an in arrays:
+                    G.set(v, np.asarray(arrays[an]))
+    for s in desc.get("synapses", []):
+        S = b["synapses"][s["name"]]
+        for v, an in s.get("variables", {}).items():
+            if v in S.syn_vars and an in arrays:
+                S.set(v, np.asarray(arrays[an]))
