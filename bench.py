@@ -91,32 +91,6 @@ def cpu_baseline(n=20000, K=1000, warm=1000, steps=2000):
                        f"{out[1][1]} in {out[1][2]:.1f} s on one; final state identical: {same}")
 
 
-class RoctxRegion:
-    """roctxProfilerResume / roctxProfilerPause around the timed region, so that a
-    `rocprofv3 --selected-regions` run traces exactly the launches the bench
-    times (outside a profiler these calls do nothing)."""
-
-    def __init__(self, on):
-        self.lib = None
-        if on:
-            import ctypes
-
-            self.lib = ctypes.CDLL("/opt/rocm/lib/librocprofiler-sdk-roctx.so.1")
-            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
-            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
-            self.lib.roctxProfilerPause(0)
-
-    def __enter__(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerResume(0)
-        return self
-
-    def __exit__(self, *exc):
-        if self.lib is not None:
-            self.lib.roctxProfilerPause(0)
-        return False
-
-
 def kernel_bytes(kind, n, events, spikes, fbytes):
     """Algorithmic bytes of one launch: neuron state read + write (v, ge, gi),
     lastspike read, not_refractory read + write; per spike the ring entry, the
@@ -336,41 +310,53 @@ def _rocprof_rows(d, suffix):
     return rows
 
 
-def summarize_trace(d):
+def summarize_trace(d, last=0):
     """Markdown table of a `rocprofv3 --kernel-trace --output-format csv` run: per
-    kernel the dispatches, total / average / median / min / max duration, and for
-    the kernel with the most GPU time the idle gaps between its launches."""
+    kernel the dispatches, total / average / median / min / max duration, over
+    every dispatch and, when last > 0, over each kernel's last `last` dispatches
+    (the timed steps of a bench run with --no-profile-pass are the step kernel's
+    last dispatches); for the kernel with the most GPU time, the idle gaps between
+    its launches in that window."""
     import numpy as np
 
     rows = _rocprof_rows(d, "kernel_trace.csv")
     by = {}
     for r in rows:
         by.setdefault(r["Kernel_Name"], []).append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    for k in by:
+        by[k].sort()
     total = sum(e - s for v in by.values() for s, e in v)
-    lines = ["| kernel | calls | total ms | avg us | median us | min us | max us | % of GPU time |",
-             "|---|---|---|---|---|---|---|---|"]
     order = sorted(by, key=lambda k: -sum(e - s for s, e in by[k]))
-    for k in order:
-        dur = np.array([e - s for s, e in by[k]], np.float64) / 1e3
-        name = k if len(k) <= 60 else k[:57] + "..."
-        lines.append(f"| `{name}` | {len(dur)} | {dur.sum() / 1e3:.3f} | {dur.mean():.2f} | {np.median(dur):.2f} | "
-                     f"{dur.min():.2f} | {dur.max():.2f} | {dur.sum() / 1e3 / (total / 1e6) * 100:.1f} |")
-    top = sorted(by[order[0]])
+    out = []
+    for title, sel in (("every dispatch", lambda v: v), (f"last {last} dispatches of each kernel",
+                                                         lambda v: v[-last:])):
+        if title.startswith("last") and last <= 0:
+            continue
+        out += [f"{title}:", "", "| kernel | calls | total ms | avg us | median us | min us | max us | % of GPU time |",
+                "|---|---|---|---|---|---|---|---|"]
+        for k in order:
+            dur = np.array([e - s for s, e in sel(by[k])], np.float64) / 1e3
+            name = k if len(k) <= 60 else k[:57] + "..."
+            out.append(f"| `{name}` | {len(dur)} | {dur.sum() / 1e3:.3f} | {dur.mean():.2f} | {np.median(dur):.2f} | "
+                       f"{dur.min():.2f} | {dur.max():.2f} | {dur.sum() / 1e3 / (total / 1e6) * 100:.1f} |")
+        out.append("")
+    top = by[order[0]][-last:] if last > 0 else by[order[0]]
     st = np.array([s for s, _ in top], np.int64)
     en = np.array([e for _, e in top], np.int64)
     if len(top) > 1:
         gap = (st[1:] - en[:-1]) / 1e3
-        lines.append("")
-        lines.append(f"`{order[0][:60]}`: start-to-start {np.mean(np.diff(st)) / 1e3:.2f} us on average; idle between "
-                     f"launches {gap.mean():.2f} us on average, {np.median(gap):.2f} us median; first start to last "
-                     f"end {(en[-1] - st[0]) / 1e6:.3f} ms")
-    return "\n".join(lines)
+        out.append(f"`{order[0][:60]}` ({len(top)} dispatches): start-to-start {np.mean(np.diff(st)) / 1e3:.2f} us on "
+                   f"average; idle between launches {gap.mean():.2f} us on average, {np.median(gap):.2f} us median; "
+                   f"first start to last end {(en[-1] - st[0]) / 1e6:.3f} ms")
+    return "\n".join(out)
 
 
-def summarize_counters(d, counter, copy_bytes=256 << 20):
+def summarize_counters(d, counter, last=0, copy_bytes=256 << 20):
     """Per-dispatch average of one counter (`rocprofv3 --pmc <counter> --output-format
-    csv`) per kernel, in KiB and in bytes scaled by what the same counter reads
-    for a device copy of copy_bytes (the copy kernel's dispatches in the run)."""
+    csv`) per kernel (over each kernel's last `last` dispatches when last > 0), in
+    KiB and in bytes scaled by what the same counter reads for a device copy of
+    copy_bytes (the largest copy-kernel dispatch of the run: bench.py
+    --copy-calibration's copies)."""
     import numpy as np
 
     rows = _rocprof_rows(d, "counter_collection.csv")
@@ -378,18 +364,22 @@ def summarize_counters(d, counter, copy_bytes=256 << 20):
     for r in rows:
         if r["Counter_Name"] != counter:
             continue
-        by.setdefault(r["Kernel_Name"], {}).setdefault(r.get("Dispatch_Id") or r.get("Correlation_Id"), 0.0)
-        by[r["Kernel_Name"]][r.get("Dispatch_Id") or r.get("Correlation_Id")] += float(r["Counter_Value"])
-    copy = [k for k in by if "copy" in k.lower()]
+        did = int(r.get("Dispatch_Id") or r.get("Correlation_Id"))
+        per = by.setdefault(r["Kernel_Name"], {})
+        per[did] = per.get(did, 0.0) + float(r["Counter_Value"])
+    copy = [k for k in by if k.startswith("__amd_rocclr_copyBuffer")]
     if not copy:
         raise SystemExit("no copy kernel in the counter run (run the bench with --copy-calibration)")
-    copy_kib = float(np.mean(list(by[copy[0]].values())))
+    copy_kib = max(max(by[k].values()) for k in copy)
     factor = copy_bytes / 1024.0 / copy_kib
-    out = {"_calibration": {"counter": counter, "copy_kernel": copy[0], "copy_bytes": copy_bytes,
-                            "copy_kib_per_dispatch": copy_kib, "factor": factor}}
+    out = {"_calibration": {"counter": counter, "copy_bytes": copy_bytes, "copy_kib_per_dispatch": copy_kib,
+                            "factor": factor}}
     for k, v in by.items():
-        kib = float(np.mean(list(v.values())))
-        out[k] = {"dispatches": len(v), "kib_per_dispatch": kib, "bytes_per_dispatch": kib * 1024.0 * factor}
+        vals = [v[i] for i in sorted(v)]
+        if last > 0:
+            vals = vals[-last:]
+        kib = float(np.mean(vals))
+        out[k[:80]] = {"dispatches": len(vals), "kib_per_dispatch": kib, "bytes_per_dispatch": kib * 1024.0 * factor}
     return out
 
 
@@ -413,18 +403,19 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="start with device copies of known size (for counter runs)")
     ap.add_argument("--sweep-steps", type=int, default=10000)
-    ap.add_argument("--roctx-region", action="store_true",
-                    help="pause profiling outside the timed region (for rocprofv3 --selected-regions)")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="print a markdown summary of a rocprofv3 kernel trace (csv) and exit")
     ap.add_argument("--summarize-counters", nargs=2, metavar=("COUNTER", "DIR"),
                     help="print per-dispatch averages of a rocprofv3 counter run (csv) as JSON and exit")
+    ap.add_argument("--last", type=int, default=0,
+                    help="with --summarize-*: also (trace) or only (counters) each kernel's last LAST dispatches")
     args = ap.parse_args()
     if args.summarize_trace:
-        print(summarize_trace(args.summarize_trace))
+        print(summarize_trace(args.summarize_trace, args.last))
         return
     if args.summarize_counters:
-        print(json.dumps(summarize_counters(args.summarize_counters[1], args.summarize_counters[0]), indent=1))
+        print(json.dumps(summarize_counters(args.summarize_counters[1], args.summarize_counters[0], args.last),
+                         indent=1))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: there is no multi-GPU path (a network would shard by "
@@ -437,10 +428,8 @@ def main():
 
     SPS = args.sim_steps_per_step
     log(f"device: {torch.cuda.get_device_name(0)}")
-    region = RoctxRegion(args.roctx_region)
     if args.copy_calibration:
-        with region:
-            copy_calibration()
+        copy_calibration()
     t0 = time.perf_counter()
     c = synth.cuba_synw_network(N=args.N, K=args.K, float_t=args.float_t, graph_steps=SPS,
                                 network_options={"fuse": not args.no_fuse})
@@ -460,11 +449,10 @@ def main():
     ev0 = net.n_events
     spk0 = int(mon.mon_n.item())
     torch.cuda.synchronize()
-    with region:
-        t0 = time.perf_counter()
-        net.run(n_sim, timing=True)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net.run(n_sim, timing=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
     gpu_ms = net.last_run_gpu_ms
     events = net.n_events - ev0
     spikes = int(mon.mon_n.item()) - spk0

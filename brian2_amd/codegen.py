@@ -42,6 +42,14 @@ def is_int(t):
     return t in ("bool", "int", "long long")
 
 
+def py_mod(a, b):
+    """Float % as Cython compiles it without cdivision (the cython target's scalar
+    blocks): fmod, then b added once when the remainder is non-zero and its sign
+    differs from b's (the same IEEE operations as the device's hb_pymod)."""
+    r = math.fmod(a, b)
+    return r + float((r != 0) & ((r < 0) ^ (b < 0))) * b
+
+
 def ctype_of(dtype, float_t="double", name=None):
     """C type of a Brian dtype name; float64 state follows the precision mode."""
     if dtype in ("float64", "double"):
@@ -172,8 +180,12 @@ class Renderer:
             if is_int(ta) and is_int(tb):
                 t = "long long" if "long long" in (ta, tb) else "int"
                 return f"hb_imod<{t}>({a}, {b})", t
+            # the cython target renders a % b as ((a % b) + b) % b (cython_generator.py:86-89),
+            # each % with Python's sign convention (cdivision=False)
             t = promote(ta, tb)
-            return f"hb_fmod_py((double)({a}), (double)({b}))", "double"
+            if t == "float":
+                return f"hb_pymodf(hb_pymodf({a}, {b}) + ({b}), {b})", "float"
+            return f"hb_pymod(hb_pymod((double)({a}), (double)({b})) + (double)({b}), (double)({b}))", "double"
         raise NotImplementedError(f"operator {op.__name__}")
 
     _F64 = {"exp": "hb_exp", "expm1": "hb_expm1", "exprel": "hb_exprel", "log": "hb_log", "sqrt": "sqrt",
@@ -292,7 +304,9 @@ class HostEval:
             if op is ast.FloorDiv:
                 return a // b if isinstance(a, int) and isinstance(b, int) else float(math.floor(a / b))
             if op is ast.Mod:
-                return a % b
+                if isinstance(a, int) and isinstance(b, int):
+                    return ((a % b) + b) % b
+                return py_mod(py_mod(float(a), float(b)) + float(b), float(b))
         if isinstance(n, ast.Call) and isinstance(n.func, ast.Name) and n.func.id in _HOST_FUNCS:
             return _HOST_FUNCS[n.func.id](*[self._e(a) for a in n.args])
         raise NotImplementedError(f"scalar expression element {ast.dump(n)}")
@@ -405,7 +419,14 @@ template <typename T> __device__ __forceinline__ T hb_floordiv(T a, T b) {
 template <typename T> __device__ __forceinline__ T hb_imod(T x, T y) {
     T r = x % y; r += ((r != 0) & ((r ^ y) < 0)) * y; return r;
 }
-__device__ __forceinline__ double hb_fmod_py(double x, double y) { return x - y * floor(1.0 * x / y); }
+// float % with Python's sign convention, as Cython compiles it without cdivision: fmod, then
+// the divisor added once when the remainder is non-zero and its sign differs from the divisor's
+__device__ __forceinline__ double hb_pymod(double a, double b) {
+    double r = fmod(a, b); r += (double)((r != 0) & ((r < 0) ^ (b < 0))) * b; return r;
+}
+__device__ __forceinline__ float hb_pymodf(float a, float b) {
+    float r = fmodf(a, b); r += (float)((r != 0) & ((r < 0) ^ (b < 0))) * b; return r;
+}
 __device__ __forceinline__ float hb_exprelf(float x) {
     if (fabsf(x) < 1e-16f) return 1.0f;
     if (x > 717.0f) return __builtin_inff();

@@ -39,9 +39,13 @@ from .codegen import (
     translate_statement,
 )
 
-KEY_TGT_SHIFT = 48  # event key: [63:48] target in tile, [47:44] rank, [43:32] delay, [31:0] CSR position
+# event key: [63:48] target in tile, [47:44] pathway rank, [43:32] push-order field, [31:0] CSR
+# position.  The push-order field is n_seg - (d - dmin) for an event of delay d (earlier pushes
+# first); 0 is kept for events carried over a change of dt (they precede every later push)
+KEY_TGT_SHIFT = 48
 MAX_RANK = 16
-MAX_SEG = 4096
+MAX_SEG = 4095
+BIG_LISTED = 64  # targets per tile and step ordered cooperatively when their events exceed cap/2
 GATHER_BLOCK = 1024
 DECL_MARK = "// ---- generated declarations ----"  # the per-kernel part of a source starts here
 
@@ -343,12 +347,54 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         "        }",
         "        __syncthreads();",
         "    }",
-        "    #pragma unroll",
-        f"    for (int q = 0; q < {npt}; q++) {{",
-        "        if (hb_bigt[q] && !hb_ovf) {",
-        "            hb_sort_idx(hb_go + hb_b[q], hb_st, hb_c[q]);",
+        "    // targets sorted in global memory (more than cap/2 events): the workgroup orders each",
+        "    // one's index list by counting, for every event, the smaller keys of the target (keys",
+        "    // staged in LDS chunks of cap), writing the ordered list to the tile's second list; the",
+        "    // owner then applies it.  Up to BIG_LISTED targets per tile take this path, further",
+        "    // ones are sorted by their owner alone.",
+        "    if (hb_anybig && !hb_ovf) {",
+        f"        int* s_bigb = s_beg; int* s_bigc = s_fill;  // free after the passes",
+        "        if (threadIdx.x == 0) s_alloc = 0;",
+        "        __syncthreads();",
+        f"        bool hb_lst[{npt}];",
+        "        #pragma unroll",
+        f"        for (int q = 0; q < {npt}; q++) {{",
+        "            hb_lst[q] = false;",
+        "            if (hb_bigt[q]) {",
+        "                const int j = atomicAdd(&s_alloc, 1);",
+        f"                if (j < {min(BIG_LISTED, TN)}) {{ s_bigb[j] = hb_b[q]; s_bigc[j] = hb_c[q]; hb_lst[q] = true; }}",
+        "            }",
+        "        }",
+        "        __syncthreads();",
+        f"        const int hb_nbig = s_alloc < {min(BIG_LISTED, TN)} ? s_alloc : {min(BIG_LISTED, TN)};",
+        "        unsigned int* hb_go2 = p.stg_ord + ((size_t)p.n_tiles + (size_t)hb_tile) * (size_t)p.tile_cap;",
+        "        for (int j = 0; j < hb_nbig; j++) {",
+        "            const int b0 = s_bigb[j], c0 = s_bigc[j];",
+        f"            for (int x0 = 0; x0 < c0; x0 += {T}) {{",
+        "                const int x = x0 + (int)threadIdx.x;",
+        "                unsigned int ixx = 0u;",
+        "                unsigned long long kx = 0ull;",
+        "                if (x < c0) { ixx = hb_go[b0 + x]; kx = hb_rkey(hb_st[ixx]); }",
+        "                int r = 0;",
+        f"                for (int y0 = 0; y0 < c0; y0 += {cap}) {{",
+        f"                    const int ny = c0 - y0 < {cap} ? c0 - y0 : {cap};",
+        "                    __syncthreads();",
+        f"                    for (int y = threadIdx.x; y < ny; y += {T}) s_key[y] = hb_rkey(hb_st[hb_go[b0 + y0 + y]]);",
+        "                    __syncthreads();",
+        "                    if (x < c0)",
+        "                        for (int y = 0; y < ny; y++) r += s_key[y] < kx ? 1 : 0;",
+        "                }",
+        "                if (x < c0) hb_go2[b0 + r] = ixx;",
+        "            }",
+        "        }",
+        "        __syncthreads();",
+        "        #pragma unroll",
+        f"        for (int q = 0; q < {npt}; q++) {{",
+        "            if (!hb_bigt[q]) continue;",
+        "            const unsigned int* hb_ol = hb_go2;",
+        "            if (!hb_lst[q]) { hb_sort_idx(hb_go + hb_b[q], hb_st, hb_c[q]); hb_ol = hb_go; }",
         "            for (int e = hb_b[q]; e < hb_b[q] + hb_c[q]; e++) {",
-        "                const unsigned int r = hb_go[e];",
+        "                const unsigned int r = hb_ol[e];",
         "                const unsigned long long hb_k = hb_rkey(hb_st[r]);",
         f"                const {vt} hb_v = hb_rval<{vt}>(hb_st[r]);",
     ]
@@ -515,6 +561,22 @@ class NeuronKernel:
             apply += ["default: break;", "}"]
             k += _fold_passes(T, U, W, self.cap, self.val_t, apply, NPT)
             k += ["    } else {", "    __syncthreads();  // the workgroup's counters, cleared at the start", "    }"]
+        if self.statemon and (not self.flush or self.statemon_when == "end"):
+            # the recorded neurons of this tile: rec_idx[hb_rlo, hb_rhi), found by two searches
+            # whose indices are the same for every thread (scalar loads), so that neurons of
+            # tiles without recorded neurons search nothing
+            k += [
+                "    int hb_rlo, hb_rhi;",
+                "    {",
+                f"        const int a_ = hb_tile * {TN}, b_ = a_ + {TN};",
+                "        int lo = 0, hi = p.n_rec;",
+                "        while (lo < hi) { const int m = (lo + hi) >> 1; if (p.rec_idx[m] < a_) lo = m + 1; else hi = m; }",
+                "        hb_rlo = __builtin_amdgcn_readfirstlane(lo);",
+                "        hi = p.n_rec;",
+                "        while (lo < hi) { const int m = (lo + hi) >> 1; if (p.rec_idx[m] < b_) lo = m + 1; else hi = m; }",
+                "        hb_rhi = __builtin_amdgcn_readfirstlane(lo);",
+                "    }",
+            ]
         # ---- per-neuron body ------------------------------------------------------
         k.append("    #pragma unroll")
         k.append(f"    for (int q = 0; q < {NPT}; q++) {{")
@@ -536,11 +598,11 @@ class NeuronKernel:
             nv = len(self.statemon)
             row = "step - p.rec_row0" if self.statemon_when == "start" else "step - 1 - p.rec_row0"
             body += [
-                f"{{ // StateMonitor (when='{self.statemon_when}')",
-                "    int lo = 0, hi = p.n_rec;",
+                f"{{ // StateMonitor (when='{self.statemon_when}'): search only the tile's recorded neurons",
+                "    int lo = hb_rlo, hi = hb_rhi;",
                 "    while (lo < hi) { const int m = (lo + hi) >> 1; if (p.rec_idx[m] < i) lo = m + 1; else hi = m; }",
                 f"    const long long row = {row};",
-                "    if (lo < p.n_rec && p.rec_idx[lo] == i && row >= 0 && row < p.rec_rows) {",
+                "    if (lo < hb_rhi && p.rec_idx[lo] == i && row >= 0 && row < p.rec_rows) {",
             ]
             for a_, v in enumerate(self.statemon):
                 body.append(f"        p.rec_buf[((size_t)row * {nv} + {a_}) * p.n_rec + lo] = (double){v};")
@@ -698,6 +760,28 @@ class NeuronKernel:
 
 
 # ---------------------------------------------------------------------------
+def onpre_lines(on_pre, target_index, syn_arrays, scalars, float_t, assign, val_t):
+    """The on_pre statements in order (synapses.py:359-383): the postsynaptic update's
+    value is assigned with `assign` (the fold applies it later), every other statement
+    runs as it is; scalars are read from the parameter struct (p.c_<name>)."""
+    types = dict(syn_arrays)
+    types["_postsynaptic_idx"] = "int"
+    types["_idx"] = "int"
+    types["t"] = "double"
+    for n, (_, ct) in scalars.items():
+        types[n] = ct
+    names = {n: f"p.c_{n}" for n in scalars}
+    lines = []
+    declared = set()
+    for i_, st in enumerate(on_pre):
+        if i_ == target_index:
+            c, _ = Renderer(types).render(st["expr"])
+            lines.append(f"{assign}({c});")
+        else:
+            lines += translate_statement(st, types, declared, float_t)
+    return _rename(lines, names)
+
+
 class GatherKernel:
     """Per-pathway event gather: spikes of step-d -> CSR segments of delay d ->
     the regions of the target tiles in the target group's staging buffer.
@@ -795,22 +879,8 @@ class GatherKernel:
         VT = self.val_t
         # the on_pre statements in order; the target update's value goes to hb_gv (the fold
         # applies it), the other statements run as they are
-        types = dict(self.syn_arrays)
-        types["_postsynaptic_idx"] = "int"
-        types["_idx"] = "int"
-        types["t"] = "double"
-        for n, (_, ct) in self.scalars.items():
-            types[n] = ct
-        names = {n: f"p.c_{n}" for n in self.scalars}
-        lines = []
-        declared = set()
-        for i_, st in enumerate(self.on_pre):
-            if i_ == self.target_index:
-                c, _ = Renderer(types).render(st["expr"])
-                lines.append(f"hb_gv[hb_u] = ({VT})({c});")
-            else:
-                lines += translate_statement(st, types, declared, self.float_t)
-        lines = _rename(lines, names)
+        lines = onpre_lines(self.on_pre, self.target_index, self.syn_arrays, self.scalars, self.float_t,
+                            f"hb_gv[hb_u] = ({VT})", VT)
         k = [source_header(GT), DECL_MARK, ps.c_decl(), self.rec_decl]
         k.append(f'extern "C" __global__ void __launch_bounds__({GT}) {self.kernel_name}(const {ps.name} p) {{')
         k += [
@@ -970,7 +1040,7 @@ class GatherKernel:
             "                    // a target outside the tiles (corrupt synapse data) is reported, not staged",
             f"                    if ((unsigned int)_postsynaptic_idx >= {NT * TT}u) {{ hb_gt[hb_u] = -1; atomicOr(p.status, 8); }}",
             f"                    hb_gk[hb_u] = ((unsigned long long)(_postsynaptic_idx - tile * {TT}) << {KEY_TGT_SHIFT})",
-            f"                        | ((unsigned long long){self.rank} << 44) | ((unsigned long long)({NS - 1} - hb_gs[hb_u]) << 32)",
+            f"                        | ((unsigned long long){self.rank} << 44) | ((unsigned long long)({NS} - hb_gs[hb_u]) << 32)",
             "                        | (unsigned long long)e;",
             "                }",
             "            }",
@@ -1067,6 +1137,91 @@ class GatherKernel:
             "#undef HB_PAIR_SRC",
             "#undef HB_PAIR_SEG",
             "    if (threadIdx.x == 0) p.events[blockIdx.x] += hb_ev;",
+            "}",
+        ]
+        self.source = "\n".join(k) + "\n"
+
+
+class CarryKernel:
+    """Events carried over a change of dt: CSpikeQueue::prepare re-bins the spikes
+    in flight (spikequeue.h:58-74), so after the change a pathway delivers, at the
+    start of each of the next few steps, a fixed list of synapses (the re-binned
+    slot) before the events of later pushes.  One thread per listed synapse: the
+    on_pre statements run as in the pathway's gather (same synapse records), and
+    the event is staged with push-order field 0 and its position in the slot's
+    list as the last key field, so that the fold applies the slot in its order and
+    before every event of the gathers."""
+
+    BLOCK = 256
+
+    def __init__(self, gather):
+        g = gather
+        self.name = g.name
+        self.kernel_name = f"carry_{g.name}"
+        ps = ParamStruct(f"hbc_{g.name}")
+        ps.ptr("clock", "long long", const=True)
+        ps.val("step_off", "long long")
+        ps.val("t0", "long long")          # the step of relative slot 0
+        ps.val("n_slots", "long long")
+        ps.ptr("slot_ptr", "unsigned int", const=True)  # n_slots + 1 offsets into pos
+        ps.ptr("pos", "unsigned int", const=True)       # CSR positions, slot by slot, in slot order
+        ps.ptr("syn_rec", "unsigned int", const=True)
+        if g.plastic:
+            ps.val("dt", "double")
+            for v, ct in g.syn_arrays.items():
+                ps.ptr(f"s_{v}", ct)
+        for n, (_, ct) in sorted(g.scalars.items()):
+            ps.val(f"c_{n}", ct)
+        ps.ptr("stg_rec", "unsigned int")
+        ps.ptr("tile_off", "unsigned int", const=True)
+        ps.ptr("tile_cnt", "unsigned int")
+        ps.val("stg_stride", "unsigned long long")
+        ps.ptr("status", "int")
+        ps.ptr("events", "unsigned long long")
+        if g.count_synapses:
+            ps.ptr("syn_count", "int")
+        self.struct = ps
+        VT, NT, TT = g.val_t, g.n_tiles, g.tile
+        lines = onpre_lines(g.on_pre, g.target_index, g.syn_arrays, g.scalars, g.float_t, f"hb_val = ({VT})", VT)
+        k = [source_header(self.BLOCK), DECL_MARK, ps.c_decl(), g.rec_decl,
+             f'extern "C" __global__ void __launch_bounds__({self.BLOCK}) {self.kernel_name}(const {ps.name} p) {{',
+             "    const long long step = p.clock[0] + p.step_off;",
+             "    const long long r = step - p.t0;",
+             "    if (r < 0 || r >= p.n_slots) return;",
+             "    const unsigned int a = p.slot_ptr[r], b = p.slot_ptr[r + 1];",
+             "    const int par = (int)(step % 3);",
+             f"    {rec_type(VT)}* hb_st = ({rec_type(VT)}*)p.stg_rec + (size_t)par * p.stg_stride;",
+             "    unsigned long long hb_ev = 0ull;",
+             "    for (unsigned int j = a + blockIdx.x * blockDim.x + threadIdx.x; j < b; j += gridDim.x * blockDim.x) {",
+             "        const unsigned int e = p.pos[j];",
+             f"        const {g.rec_name} hb_r = ((const {g.rec_name}*)p.syn_rec)[e];",
+             "        const int _postsynaptic_idx = hb_r.post;",
+             "        const int _idx = (int)e;",
+             "        (void)_idx;",
+             f"        {VT} hb_val;"]
+        if g.plastic:
+            k.append("        const double t = (double)step * p.dt;")
+            k += [f"        {ct} {v} = p.s_{v}[e];" for v, ct in g.syn_arrays.items()]
+        else:
+            k += [f"        const {ct} {v} = hb_r.{v};" for v, ct in g.syn_arrays.items()]
+        k += ["        " + x for x in lines]
+        k += [f"        p.s_{v}[e] = {v};" for v in g.syn_arrays if v in g.plastic]
+        k += [
+            f"        if ((unsigned int)_postsynaptic_idx >= {NT * TT}u) {{ atomicOr(p.status, 8); continue; }}",
+            f"        const int tile = _postsynaptic_idx / {TT};",
+            f"        const unsigned long long key = ((unsigned long long)(_postsynaptic_idx - tile * {TT}) << {KEY_TGT_SHIFT})",
+            f"            | ((unsigned long long){g.rank} << 44) | (unsigned long long)(j - a);",
+            f"        const unsigned int c = atomicAdd(p.tile_cnt + (size_t)par * {NT} + tile, 1u);",
+            "        const unsigned int lo = p.tile_off[tile], hi = p.tile_off[tile + 1];",
+            "        if (c >= hi - lo) { atomicOr(p.status, 1); continue; }",
+            "        hb_rset(hb_st[lo + c], key, hb_val);",
+        ]
+        if g.count_synapses:
+            k.append("        p.syn_count[e] += 1;  // a slot lists a synapse at most once")
+        k += [
+            "        hb_ev++;",
+            "    }",
+            "    if (hb_ev) atomicAdd(p.events, hb_ev);",
             "}",
         ]
         self.source = "\n".join(k) + "\n"

@@ -263,10 +263,10 @@ class SpikeGeneratorGroup(_GroupBase):
         self.N = int(N)
         self.name = name or _unique("spikegeneratorgroup")
         idx = np.asarray(indices, np.int32)
+        self._idx_in = idx
+        self.times = None if times is None else np.asarray(times, np.float64)
         if timebins is None:
-            # spikegeneratorgroup.py:241-245: shift by 1e-3 dt, then truncate
-            times = np.asarray(times, np.float64)
-            timebins = np.asarray((times + 1e-3 * dt) / dt, dtype=np.int32)
+            timebins = self.bins(dt)
         tb = np.asarray(timebins, np.int32)
         order = np.lexsort((idx, tb))
         self.indices, self.timebins = idx[order], tb[order]
@@ -275,6 +275,24 @@ class SpikeGeneratorGroup(_GroupBase):
         self._dev = None
         self.net = None
         self.uses_refractory = False
+
+
+def _spikegen_bins(self, dt):
+    """spikegeneratorgroup.py:241-245: times shifted by 1e-3 dt, then truncated."""
+    return np.asarray((self.times + 1e-3 * dt) / dt, dtype=np.int32)
+
+
+def _spikegen_rebin(self, dt):
+    """The bins at a new dt (before_run recomputes them when dt changed)."""
+    if self.times is None:
+        raise NotImplementedError(f"{self.name}: spikes given as time bins cannot follow a change of dt")
+    tb = self.bins(dt)
+    order = np.lexsort((self._idx_in, tb))
+    self.indices, self.timebins = self._idx_in[order], tb[order]
+
+
+SpikeGeneratorGroup.bins = _spikegen_bins
+SpikeGeneratorGroup.rebin = _spikegen_rebin
 
 
 class PoissonGroup(_GroupBase):
@@ -341,6 +359,9 @@ class Synapses:
         self.syn = {}
         self.csr = None
         self.net = None
+        self._queue_size = 1      # CSpikeQueue's queue length (spikequeue.h:30-36, 131-149)
+        self.carry = None         # events carried over a change of dt (see Network.run)
+        self._events_base = 0     # events delivered by earlier builds of the network
 
     @staticmethod
     def _synaptic_vars(spec, skip, what):
@@ -437,6 +458,12 @@ class Synapses:
             raise RuntimeError("connect() first")
         a = np.broadcast_to(np.asarray(values, np.float64), (len(self.pre),)).copy()
         self.syn[name] = a
+        if self.csr is not None and name in self.csr["syn"]:
+            # built: the device copy is in CSR storage order
+            t = self.csr["syn"][name]
+            order = self.csr.get("order")
+            v = a[order] if order is not None else a
+            t[: self.csr["n_syn"]].copy_(dev(v.astype(np.float32 if t.dtype == _torch().float32 else np.float64)))
 
     def set_delay(self, delay):
         self.delay = delay
@@ -477,6 +504,8 @@ class Synapses:
         dmin = int(dsteps.min()) if S else 0
         dmax = int(dsteps.max()) if S else 0
         n_seg = dmax - dmin + 1
+        self._dsteps = dsteps
+        self._queue_size = max(self._queue_size, dmax + 1)  # ensure_delay (spikequeue.h:143-149)
         nsrc = self.src_range[1] - self.src_range[0]
         seg = (self.pre.astype(np.int64) - self.src_range[0]) * n_seg + (dsteps - dmin)
         order = np.lexsort((np.arange(S), seg))  # per (source, delay): synapse order
@@ -515,7 +544,10 @@ class Synapses:
     @property
     def n_events(self):
         """Synapse ids delivered so far (Brian's 'synaptic events', synapses.pyx:42-50)."""
-        return int(self.events_dev.sum().item())
+        n = int(self.events_dev.sum().item()) + self._events_base
+        if getattr(self, "carry_events", None) is not None:
+            n += int(self.carry_events.item())
+        return n
 
     def event_counts(self):
         """Per-synapse delivered-event counts in synapse order (count_events=True)."""
@@ -545,6 +577,7 @@ class SpikeMonitor:
         self.name = name or _unique("spikemonitor")
         self._i = []
         self._step = []
+        self._t = []  # times of the records (step * dt of the run that recorded them)
         self.count_dev = None
         self.net = None
 
@@ -564,13 +597,21 @@ class SpikeMonitor:
     def t(self, dt):
         return self.step * dt
 
-    def _sorted(self):
+    @property
+    def t_(self):
+        """Spike times in seconds (step * dt of the run that recorded them)."""
+        return self._sorted(with_t=True)[2]
+
+    def _sorted(self, with_t=False):
         if self.net is not None:
             self.net._drain_monitors()
         i = np.concatenate(self._i) if self._i else np.zeros(0, np.int32)
         s = np.concatenate(self._step) if self._step else np.zeros(0, np.int64)
-        o = np.lexsort((i, s))  # spikes of one step are ascending (threshold.pyx:18-32)
-        return i[o], s[o]
+        t = np.concatenate(self._t) if self._t else np.zeros(0, np.float64)
+        # by time (steps of runs with different dt do not compare), then index: spikes of one
+        # step are ascending (threshold.pyx:18-32)
+        o = np.lexsort((i, t))
+        return (i[o], s[o], t[o]) if with_t else (i[o], s[o])
 
     @property
     def count(self):
@@ -595,7 +636,13 @@ class StateMonitor:
         self.sorted_idx = np.unique(rec)
         self.name = name or _unique("statemonitor")
         self._rows = []
+        self._times = []  # per run: the recorded steps' times
         self.net = None
+
+    @property
+    def t_(self):
+        """Times of the recorded rows (seconds)."""
+        return np.concatenate(self._times) if self._times else np.zeros(0)
 
     def values(self):
         """Array [n_vars, n_record, n_steps] in the order of `record`."""
@@ -620,6 +667,7 @@ class PopulationRateMonitor:
         self.name = name or _unique("ratemonitor")
         self._counts = []
         self._steps = []
+        self._dts = []  # per run: dt of its steps
         self.net = None
 
     @property
@@ -634,9 +682,19 @@ class PopulationRateMonitor:
     def t(self, dt):
         return self.step * dt
 
-    def rate(self, dt):
-        """Hz, as ratemonitor.pyx:47: _num_spikes / _clock_dt / _num_source_neurons."""
+    @property
+    def t_(self):
+        """Times of the recorded steps (seconds, with the dt of their run)."""
+        if not self._dts:
+            return np.zeros(0)
+        return np.concatenate([st * d[0] if len(d) else st * 0.0 for st, d in zip(self._steps, self._dts)])
+
+    def rate(self, dt=None):
+        """Hz, as ratemonitor.pyx:47: _num_spikes / _clock_dt / _num_source_neurons (dt:
+        that of each step's run when not given)."""
         n = self.range[1] - self.range[0]
+        if dt is None:
+            dt = np.concatenate(self._dts) if self._dts else np.zeros(0)
         return self.count.astype(np.float64) / dt / n
 
     def smooth_rate(self, dt, window="gaussian", width=None):
@@ -821,6 +879,8 @@ class Network:
         if self.concurrent_gathers:
             for i, s in enumerate([s for s in paths if s.fused_into is None]):
                 _lib.check(L.hb_plan_set_lane(self._plan, s.launch_index, 1 + i % 3), "hb_plan_set_lane")
+                if s.carry is not None:
+                    _lib.check(L.hb_plan_set_lane(self._plan, s.carry_index, 1 + i % 3), "hb_plan_set_lane")
         self._built = True
 
     def _alloc_staging(self, g, incoming):
@@ -867,8 +927,9 @@ class Network:
         free, _ = torch.cuda.mem_get_info()
         budget = int(free * self.bucket_mem_frac)
         big = need_target > g.lds_cap // 2
-        g.tile_cap = int(0 if not big else min(need_tile, (budget // 4) // (4 * g.n_tiles)))
-        stg_budget = budget - 4 * g.n_tiles * g.tile_cap
+        # two index lists per tile (a target's events, then the same ordered by key)
+        g.tile_cap = int(0 if not big else min(need_tile, (budget // 4) // (8 * g.n_tiles)))
+        stg_budget = budget - 8 * g.n_tiles * g.tile_cap
         limit = int(min(total, 2**32 - 1, stg_budget // (3 * (8 + vbytes))))
         region = indeg_t if limit >= total else (indeg_t * limit) // max(total, 1)
         off = np.zeros(g.n_tiles + 1, np.int64)
@@ -877,7 +938,7 @@ class Network:
         g.tile_off = dev(off.astype(np.uint32).view(np.int32))
         g.tile_cnt = zeros(3 * g.n_tiles, "int")
         g.stg_rec = torch.empty(3 * g.stg_cap * rec_words(g.val_t), dtype=torch.int32, device="cuda")
-        g.stg_ord = torch.empty(max(1, g.n_tiles * g.tile_cap), dtype=torch.int32, device="cuda")
+        g.stg_ord = torch.empty(max(1, 2 * g.n_tiles * g.tile_cap), dtype=torch.int32, device="cuda")
 
     def _gather_units(self, s, window):
         """(delay, spike) pairs per gather unit: about one window of events per unit on
@@ -990,6 +1051,11 @@ class Network:
             for s in g.incoming:
                 s.launch_index = g.launch_index
             self._add_launch(g.fn, F.grid, F.block, self._launch_args(g), g.fused_cls)
+            # carried ids after the fused launch of the step (its gathers and theirs write the
+            # same staging buffer; the keys order them)
+            for s in g.incoming:
+                if s.carry is not None:
+                    self._build_carry(s)
         else:
             self._add_launch(g.fn, (g.N + g.tile - 1) // g.tile, g.block, self._group_args(g), g.struct_cls)
         end_mon = g.stmon is not None and g.stmon.when == "end"
@@ -1099,6 +1165,8 @@ class Network:
         fn = ctypes.c_void_p()
         _lib.check(L.hb_spikegen_function(ctypes.byref(fn)))
         g.fn = fn
+        if g.times is not None:
+            g.rebin(self.dt)
         g.gen_tb = dev(g.timebins, "int")
         g.gen_idx = dev(g.indices, "int")
         g.gen_state = zeros(1, "long long")
@@ -1165,6 +1233,8 @@ class Network:
 
     def _build_gather(self, s):
         self._gather_setup(s)
+        if s.carry is not None:
+            self._build_carry(s)  # before the gather: both write s's synaptic variables and counters
         s.fn = self.device.function(s.kernel.source, s.kernel.kernel_name)
         s.events_dev = _torch().zeros(s.grid, dtype=_torch().int64, device="cuda")
         s.args.events = s.events_dev.data_ptr()
@@ -1344,6 +1414,8 @@ class Network:
                 out.append((s.csr["syn"], n))
             if getattr(s, "syn_count_dev", None) is not None:
                 out.append((s, "syn_count_dev"))
+            if s.carry is not None:
+                out.append((s, "carry_events"))
         return out
 
     def store(self, name="default"):
@@ -1361,13 +1433,13 @@ class Network:
                 snap["state"][id(g)] = {n: t.clone() for n, t in g._dev.items()}
             m = getattr(g, "smon", None)
             if m is not None:
-                snap["monitors"][id(m)] = (list(m._i), list(m._step), m._drained)
+                snap["monitors"][id(m)] = (list(m._i), list(m._step), list(m._t), m._drained)
             m = getattr(g, "stmon", None)
             if m is not None:
-                snap["monitors"][id(m)] = list(m._rows)
+                snap["monitors"][id(m)] = (list(m._rows), list(m._times))
             m = getattr(g, "rmon", None)
             if m is not None:
-                snap["monitors"][id(m)] = (list(m._counts), list(m._steps))
+                snap["monitors"][id(m)] = (list(m._counts), list(m._steps), list(m._dts))
         if not hasattr(self, "_stored"):
             self._stored = {}
         self._stored[name] = snap
@@ -1383,25 +1455,202 @@ class Network:
                     t.copy_(snap["state"][id(g)][n])
             m = getattr(g, "smon", None)
             if m is not None:
-                i, s, d = snap["monitors"][id(m)]
-                m._i, m._step, m._drained = list(i), list(s), d
+                i, s, t, d = snap["monitors"][id(m)]
+                m._i, m._step, m._t, m._drained = list(i), list(s), list(t), d
             m = getattr(g, "stmon", None)
             if m is not None:
-                m._rows = list(snap["monitors"][id(m)])
+                r, t = snap["monitors"][id(m)]
+                m._rows, m._times = list(r), list(t)
             m = getattr(g, "rmon", None)
             if m is not None:
-                c, st = snap["monitors"][id(m)]
-                m._counts, m._steps = list(c), list(st)
+                c, st, d = snap["monitors"][id(m)]
+                m._counts, m._steps, m._dts = list(c), list(st), list(d)
         self.t_step = snap["t_step"]
 
+    # -- a change of dt between runs ---------------------------------------------
+    EPSILON_DT = 1e-4  # Clock.epsilon_dt (core/clocks.py:290)
+
+    @classmethod
+    def calc_timestep(cls, t, dt):
+        """Clock._calc_timestep (core/clocks.py:350-371): round(t / dt), or ceil(t / dt)
+        when that step's time is not within epsilon_dt * dt of t."""
+        new_i = int(np.int64(np.round(t / dt)))
+        new_t = new_i * dt
+        if new_t == t or abs(new_t - t) / dt <= cls.EPSILON_DT:
+            return new_i
+        return int(np.int64(np.ceil(t / dt)))
+
+    def _pending_slots(self, s):
+        """The synapse ids in s's spike queue after the last step that ran, by relative
+        slot (slot r is delivered at step t_step + r), each in the reference queue's
+        order: ids carried from an earlier change of dt first, then by push step,
+        spiking source (ascending) and synapse index (CSpikeQueue::push,
+        spikequeue.h:151-191).  Returns a list of s._queue_size int64 arrays."""
+        now = self.t_step
+        Q = s._queue_size
+        parts = [[] for _ in range(Q)]
+        c = s.carry
+        if c is not None:
+            for rc in range(len(c["ptr"]) - 1):
+                r = c["t0"] + rc - now
+                if 0 <= r < Q and c["ptr"][rc + 1] > c["ptr"][rc]:
+                    parts[r].append(c["syn"][c["ptr"][rc]:c["ptr"][rc + 1]])
+        g = s.source
+        P = len(g.ring_parts)
+        cnt = g.ring_cnt.cpu().numpy().reshape(g.ring_slots, P)
+        spk = g.ring_spk.cpu().numpy().reshape(g.ring_slots, g.N)
+        bounds = list(g.ring_parts) + [g.N]
+        a, b = s.src_range
+        d = s._dsteps
+        dmax = int(d.max()) if len(d) else 0
+        by_src = np.argsort(s.pre, kind="stable")  # each source's synapses in synapse order
+        first = np.searchsorted(s.pre[by_src], np.arange(g.N + 1))
+        ev_r, ev_syn = [], []
+        for st in range(max(0, now - dmax), now):  # push steps, oldest first
+            slot = st % g.ring_slots
+            x = np.concatenate([spk[slot, bounds[p]:bounds[p] + cnt[slot, p]] for p in range(P)])
+            x = np.sort(x[(x >= a) & (x < b)])
+            for i in x:
+                syn = by_src[first[i]:first[i + 1]]
+                r = st + d[syn] - now
+                keep = (r >= 0) & (r < Q)
+                ev_r.append(r[keep])
+                ev_syn.append(syn[keep])
+        if ev_r:
+            r = np.concatenate(ev_r)
+            syn = np.concatenate(ev_syn)
+            o = np.argsort(r, kind="stable")
+            r, syn = r[o], syn[o]
+            cuts = np.searchsorted(r, np.arange(Q + 1))
+            for k in range(Q):
+                if cuts[k + 1] > cuts[k]:
+                    parts[k].append(syn[cuts[k]:cuts[k + 1]])
+        return [np.concatenate(p_).astype(np.int64) if p_ else np.zeros(0, np.int64) for p_ in parts]
+
+    def _change_dt(self, new_dt):
+        """A run with another dt (Network.run after defaultclock.dt changed): the clock
+        moves to the same time at the new dt (Clock._set_t_update_dt,
+        core/clocks.py:373-400); every pathway re-bins its spikes in flight the way
+        CSpikeQueue::prepare does (spikequeue.h:58-74: the ids of relative slot i move
+        to slot (int)(i * old_dt / new_dt + 0.5) of a queue of (int)(size * old_dt /
+        new_dt) + 1 slots, a later slot replacing an earlier one that maps to the same
+        place), and the network is built again for the new dt (delay steps, integrator
+        constants, spike-generator bins, Poisson probabilities) with its state, counters
+        and monitors carried over.  The re-binned ids are delivered by the pathways'
+        carry kernels before every later push."""
+        new_dt = float(new_dt)
+        old_dt = self.dt
+        new_step = self.calc_timestep(self.t_step * old_dt, new_dt)
+        if not self._built:
+            self.dt, self.t_step = new_dt, new_step
+            return
+        torch = _torch()
+        all_syn = [o for o in self.objs if isinstance(o, Synapses)]
+        for S in all_syn:
+            if S.post_spec is not None:
+                raise NotImplementedError(f"{S.name}: a change of dt with an on_post pathway")
+            if S.spec is not None and getattr(S, "_dsteps", None) is None:
+                raise NotImplementedError(f"{S.name}: a change of dt needs connectivity given by connect(i, j)")
+        for g in self.groups:
+            if isinstance(g, SpikeGeneratorGroup) and g.times is None:
+                raise NotImplementedError(f"{g.name}: spikes given as time bins cannot follow a change of dt")
+        torch.cuda.synchronize()
+        self._drain_monitors()
+        factor = old_dt / new_dt
+        for S in self.paths:
+            slots = self._pending_slots(S)
+            newsize = int(S._queue_size * factor) + 1
+            new = [np.zeros(0, np.int64)] * newsize
+            for i, ids in enumerate(slots):
+                k = int(i * factor + 0.5)
+                if k < newsize:  # a slot mapped past the end is lost, as in the reference
+                    new[k] = ids
+            S._queue_size = newsize
+            while len(new) > 1 and len(new[-1]) == 0:
+                new.pop()
+            ptr = np.zeros(len(new) + 1, np.int64)
+            np.cumsum([len(x) for x in new], out=ptr[1:])
+            S.carry = dict(t0=new_step, ptr=ptr, syn=np.concatenate(new)) if ptr[-1] else None
+            # state that outlives the build: counters, synaptic variables the code writes
+            S._events_base = S.n_events
+            if S.count_events:
+                S._count_host = S.event_counts()
+            for n in S.plastic:
+                S.syn[n] = S.get(n)
+        for g in self.groups:
+            if isinstance(g, NeuronGroup):
+                g._host = {n: g.get(n) for n in g.vars}
+                g._dev = None
+            elif isinstance(g, SpikeGeneratorGroup):
+                g.rebin(new_dt)
+            m = getattr(g, "smon", None)
+            if m is not None:
+                m._count_host = m.count
+        _lib.check(L.hb_plan_destroy(self._plan), "hb_plan_destroy")
+        self._plan = None
+        for S in all_syn:
+            S.csr = None
+            S.syn_count_dev = None
+            S.carry_events = None  # counted in _events_base; a new carry brings its own counter
+        self.dt, self.t_step = new_dt, new_step
+        self._built = False
+        self._stored = {}  # snapshots of the previous build do not apply to the new one
+        self._build()
+        for S in self.paths:
+            if getattr(S, "_count_host", None) is not None:
+                S.syn_count_dev[: S.csr["n_syn"]].copy_(dev(S._count_host[S.csr["order"]].astype(np.int32)))
+                S._count_host = None
+        for g in self.groups:
+            m = getattr(g, "smon", None)
+            if m is not None and getattr(m, "_count_host", None) is not None:
+                m.count_dev[: len(m._count_host)].copy_(dev(m._count_host.astype(np.int32)))
+                m._count_host = None
+
+    def _build_carry(self, s):
+        """The launch delivering s's carried ids (after a change of dt): CSR positions of
+        the listed synapses, slot by slot, for kernels.CarryKernel."""
+        c = s.carry
+        inv = np.empty(s.csr["n_syn"], np.int64)
+        inv[s.csr["order"]] = np.arange(s.csr["n_syn"])
+        s.carry_pos = dev(inv[c["syn"]].astype(np.uint32).view(np.int32))
+        s.carry_ptr = dev(c["ptr"].astype(np.uint32).view(np.int32))
+        s.carry_events = _torch().zeros(1, dtype=_torch().int64, device="cuda")
+        K = _kernels.CarryKernel(s.kernel)
+        fn = self.device.function(K.source, K.kernel_name)
+        cls = K.struct.ctypes_struct()
+        g = s.target
+        v = {"clock": self.clock.data_ptr(), "step_off": 0, "t0": c["t0"], "n_slots": len(c["ptr"]) - 1,
+             "slot_ptr": s.carry_ptr.data_ptr(), "pos": s.carry_pos.data_ptr(), "syn_rec": s.csr["rec"].data_ptr(),
+             "stg_rec": g.stg_rec.data_ptr(), "tile_off": g.tile_off.data_ptr(), "tile_cnt": g.tile_cnt.data_ptr(),
+             "stg_stride": g.stg_cap, "status": self.status.data_ptr(), "events": s.carry_events.data_ptr()}
+        for n, (val, ct) in s.kernel.scalars.items():
+            v[f"c_{n}"] = val
+        if s.plastic:
+            v["dt"] = self.dt
+            for n in s.syn_vars:
+                v[f"s_{n}"] = s.csr["syn"][n].data_ptr()
+        if s.count_events:
+            v["syn_count"] = s.syn_count_dev.data_ptr()
+        a = cls()
+        for name, _ in cls._fields_:
+            if name in v:
+                setattr(a, name, v[name])
+        most = int(np.diff(c["ptr"]).max())
+        grid = max(1, min(4 * self.device.n_cu, -(-most // K.BLOCK)))
+        s.carry_index = len(self._launches)
+        self._add_launch(fn, grid, K.BLOCK, a, cls)
+
     # -- run ------------------------------------------------------------------
-    def run(self, n_steps=None, duration=None, graph_steps=None, timing=False):
+    def run(self, n_steps=None, duration=None, graph_steps=None, timing=False, dt=None):
         """Advances the network by n_steps time steps (or duration seconds).
 
-        timing: record HIP events on the launch stream around every enqueued chunk of
-        steps; `last_run_gpu_ms` is then the device time of the step launches (the end-of-
-        run fold excluded), without the host round trips between chunks."""
+        dt: the time step of this run, when it differs from the previous run's (see
+        _change_dt); timing: record HIP events on the launch stream around every enqueued
+        chunk of steps; `last_run_gpu_ms` is then the device time of the step launches (the
+        end-of-run fold excluded), without the host round trips between chunks."""
         torch = _torch()
+        if dt is not None and float(dt) != self.dt:
+            self._change_dt(dt)
         if n_steps is None:
             n_steps = int(round(duration / self.dt))
         n_steps = int(n_steps)
@@ -1462,6 +1711,10 @@ class Network:
             if m is not None and n_steps > 0:
                 nv, nr = len(m.variables), len(m.sorted_idx)
                 m._rows.append(m.rec_buf.view(n_steps, nv, nr).cpu().numpy().copy())
+                # rows of steps t_step - n .. t_step - 1, stamped with the step's time (the
+                # template records the clock's t, statemonitor.pyx:6-38)
+                first = self.t_step - n_steps
+                m._times.append(np.arange(first, first + n_steps, dtype=np.float64) * self.dt)
             m = getattr(g, "rmon", None)
             if m is not None and n_steps > 0:
                 first = self.t_step - n_steps
@@ -1471,6 +1724,7 @@ class Network:
                     counts = m.buf[:n_steps].cpu().numpy().astype(np.int64)
                 m._counts.append(counts)
                 m._steps.append(np.arange(first, first + n_steps, dtype=np.int64))
+                m._dts.append(np.full(n_steps, self.dt))
         st = int(self.status.cpu().item())
         if st:
             raise _lib.HipBrianError(self._status_message(st))
@@ -1566,4 +1820,5 @@ class Network:
             if n > m._drained:
                 m._i.append(m.mon_i[m._drained:n].cpu().numpy().copy())
                 m._step.append(m.mon_step[m._drained:n].cpu().numpy().astype(np.int64))
+                m._t.append(m._step[-1] * self.dt)
                 m._drained = n

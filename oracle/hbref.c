@@ -253,6 +253,10 @@ int hbr_queue_prepare(hbr_queue* q, const double* delays, int64_t n_delays, cons
         for (int64_t i = 0; i < oldsize; i++) {
             ivec* src = &q->ring[(i + q->offset) % oldsize];
             int64_t tgt = (int64_t)(int)((double)i * factor + 0.5);
+            /* with a factor below 0.5 the last slots can map to newsize itself: the reference
+               then assigns past the vector's size (its content is never peeked and is
+               overwritten when the queue grows), i.e. those spikes are lost */
+            if (tgt >= newsize) continue;
             free(r[tgt].v); /* assignment replaces the slot */
             r[tgt].n = r[tgt].cap = src->n;
             r[tgt].v = src->n ? (int32_t*)malloc(sizeof(int32_t) * (size_t)src->n) : NULL;

@@ -169,6 +169,9 @@ class Queue:
     def advance(self):
         self.L.hbr_queue_advance(self.q)
 
+    def size(self):
+        return int(self.L.hbr_queue_size(self.q))
+
     def __del__(self):
         if getattr(self, "q", None):
             self.L.hbr_queue_destroy(self.q)
@@ -386,6 +389,42 @@ def build_cuba_delay(meta):
     Ce = OraclePathway(G, G, pre[exc], post[exc], 1, w=w32[exc].astype(np.float64), delay=d[exc] * dt)
     Ci = OraclePathway(G, G, pre[~exc], post[~exc], 2, w=w32[~exc].astype(np.float64), delay=d[~exc] * dt)
     return G, [Ci, Ce]
+
+
+def calc_timestep(t, dt):
+    """Clock._calc_timestep (brian2/core/clocks.py:350-371), epsilon_dt = 1e-4."""
+    new_i = int(np.int64(np.round(t / dt)))
+    if new_i * dt == t or abs(new_i * dt - t) / dt <= 1e-4:
+        return new_i
+    return int(np.int64(np.ceil(t / dt)))
+
+
+def dt_change_runs(meta):
+    """The runs of the dt-change configuration as Brian's clock takes them: per run
+    (dt, first step, steps).  dt = (k * 0.1) * ms and each run's end time t_prev + dur
+    * ms as the script computes them (Network.run keeps t_end, network.py:1254-1263;
+    Clock.set_interval starts at calc_timestep(t, dt), clocks.py:402-429)."""
+    out, t = [], 0.0
+    for k, dur in meta["runs"]:
+        dt = (k * 0.1) * 0.001
+        t_end = t + dur * 0.001
+        a, b = calc_timestep(t, dt), calc_timestep(t_end, dt)
+        out.append((dt, a, b - a))
+        t = t_end
+    return out
+
+
+def run_dt_changes(meta):
+    """The oracle over the dt-change configuration's runs, queues kept across runs.
+    Returns (spike indices, spike times, group)."""
+    G, paths = build_cuba_delay(meta)
+    qs = OracleQueues(len(paths))
+    ii, tt = [], []
+    for dt, a, n in dt_change_runs(meta):
+        i, st, _ = run([G], paths, dt, n, start_step=a, queues=qs)
+        ii.append(i)
+        tt.append(st * dt)
+    return np.concatenate(ii), np.concatenate(tt), G
 
 
 LIF = dict(N=1000, NE=800, n_in=200, tau_m=0.02, tau_s=0.005, theta=0.01, bias=0.0105, p=0.1,

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from brian2_amd import codegen as cg
-from brian2_amd.kernels import GatherKernel, NeuronKernel, scalar_values
+from brian2_amd.kernels import CarryKernel, GatherKernel, NeuronKernel, scalar_values
 from brian2_amd.runtime import Model, NeuronGroup
 
 
@@ -134,6 +134,10 @@ def test_generated_kernels_compile_for_gfx950(model, float_t, tile):
                       scalar_values(spec, {"dt": 1e-4}, float_t), "double", float_t, "ge", n_tiles, tile)
     _no_float_atomics(gk.source)
     _compile(gk.source)
+    # the kernel that delivers events carried over a change of dt, from the gather's records
+    ck = CarryKernel(gk)
+    _no_float_atomics(ck.source)
+    _compile(ck.source)
 
 
 def test_gather_kernel_limits():
@@ -276,3 +280,39 @@ def test_split_spike_ring_kernels_compile_for_gfx950():
     with pytest.raises(ValueError):
         NeuronKernel("g", N, G.vars, "double", G.su, G.th, G.rs, "fused", _scalars(G), [], cap, "float",
                      tile=1024, ring_parts=(10, 4000))
+
+
+# float % and // as the cython target evaluates them (cython_generator.py:86-89 renders
+# a % b as ((a % b) + b) % b; Cython compiles % of doubles without cdivision as fmod plus
+# a sign correction, and // of doubles as floor(a / b))
+MOD_CASES = [(-7.5, 2.0), (7.5, -2.0), (-7.5, -2.0), (7.5, 2.0), (-1e-17, 1.0), (1e-17, -1.0), (5.0, 2.5),
+             (-5.0, 2.5), (0.1, 0.03), (-0.1, 0.03), (1e300, 3.0), (-2.5e-3, 1e-3), (3.0, 1e-300)]
+
+
+def test_py_mod_is_cpythons_float_mod_where_zero_signs_agree():
+    import math
+
+    rng = np.random.default_rng(2)
+    pairs = MOD_CASES + [(float(a), float(b)) for a, b in rng.standard_normal((500, 2)) * [[100.0, 3.0]]]
+    for a, b in pairs:
+        r = cg.py_mod(a, b)
+        want = a % b  # CPython's float modulo (the same algorithm up to the sign of a zero result)
+        assert r == want or (r == 0 and want == 0), (a, b, r, want)
+        assert math.copysign(1.0, r) == math.copysign(1.0, b) or r == 0, (a, b)
+    # the known discontinuity: -1e-17 % 1.0 rounds to 1.0; the cython rendering applies % twice
+    assert cg.py_mod(-1e-17, 1.0) == 1.0
+    assert cg.HostEval({"a": -1e-17, "b": 1.0}).eval("a % b") == 0.0
+    assert cg.HostEval({"a": -7.5, "b": 2.0}).eval("a % b") == 0.5
+    assert cg.HostEval({"a": -7, "b": 3}).eval("a % b") == 2
+    assert cg.HostEval({"a": -7.5, "b": 2.0}).eval("a // b") == -4.0
+
+
+def test_mod_renders_as_the_cython_target():
+    c, t = R("a % b", a="double", b="double")
+    assert c.count("hb_pymod(") == 2 and t == "double"
+    c, t = R("a % b", a="float", b="float")
+    assert c.count("hb_pymodf(") == 2 and t == "float"
+    c, t = R("i % j", i="int", j="int")
+    assert "hb_imod" in c and t == "int"
+    c, t = R("a // b", a="double", b="double")
+    assert c.startswith("floor(")

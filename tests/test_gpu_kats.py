@@ -68,3 +68,35 @@ def test_device_transmission_kats_in_two_runs():
     net.run(1)
     net.run(3)
     assert np.array_equal(mon.values()[0, 0], [3, 12, 33, 48])
+
+
+def test_device_subgroup_propagation_kat():
+    """test_subgroup.py:718-729 (test_synaptic_propagation): G1 (10 neurons, v > 1, reset
+    v = 0) with the odd neurons at v = 1.1; Synapses(G1[1:6], G2[10:], on_pre='v += 1'),
+    connect(i == j); after one step and the propagation, G2.v is 1 at 10, 12, 14."""
+    from brian2_amd import Model, Network, NeuronGroup, Synapses
+    from brian2_amd.kernels import CodeSpec
+
+    v1 = {"kind": "ArrayVariable", "dtype": "float64", "owner": "g1"}
+    th = {"name": "g1_th", "template": "threshold", "owner": "g1",
+          "blocks": {"None": {"scalar": [], "conditional_write": {},
+                              "vector": [{"var": "_cond", "op": ":=", "expr": "v > 1", "dtype": "bool"}]}},
+          "variables": {"_cond": {"kind": "AuxiliaryVariable", "dtype": "bool"}, "v": v1},
+          "indices": {"_cond": "_idx", "v": "_idx"}}
+    rs = {"name": "g1_rs", "template": "reset", "owner": "g1",
+          "blocks": {"None": {"scalar": [], "conditional_write": {},
+                              "vector": [{"var": "v", "op": "=", "expr": "0", "dtype": "float64"}]}},
+          "variables": {"v": v1}, "indices": {"v": "_idx"}}
+    G1 = NeuronGroup(10, Model({"group": {"threshold": th, "reset": rs}}), name="g1", variables={"v": "float64"})
+    v0 = np.zeros(10)
+    v0[1::2] = 1.1  # G1.v['i%2==1'] = 1.1
+    G1.v = v0
+    G2 = NeuronGroup(20, variables={"v": "float64"})
+    S = Synapses(G1[1:6], G2[10:], onpre_spec("1"))
+    S.connect(np.arange(5), np.arange(5))  # i == j within the subgroups
+    net = Network(G1, G2, S, dt=kats.DT)
+    net.run(1)
+    expected = np.zeros(20)
+    expected[[10, 12, 14]] = 1
+    assert np.array_equal(G2.get("v"), expected)
+    assert isinstance(S.spec, CodeSpec)

@@ -313,6 +313,66 @@ def test_oracle_queue_vs_reference_cspikequeue():
         R.csq_destroy(rq)
 
 
+@pytest.mark.skipif(not os.path.exists(REF_CSQ), reason="oracle/_ref not built (needs the reference tree)")
+def test_oracle_queue_dt_changes_vs_reference_cspikequeue():
+    """Random push sequences with dt changed between runs (re-preparing with the same
+    delays in seconds), against the reference's CSpikeQueue: the re-binning of queued
+    spikes (spikequeue.h:58-74), including factors below 1 where later slots replace
+    earlier ones."""
+    R = ctypes.CDLL(REF_CSQ)
+    R.csq_create.restype = ctypes.c_void_p
+    R.csq_create.argtypes = [ctypes.c_int, ctypes.c_int]
+    R.csq_destroy.argtypes = [ctypes.c_void_p]
+    R.csq_prepare.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_int32), ctypes.c_int, ctypes.c_double]
+    R.csq_push.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+    R.csq_peek.restype = ctypes.c_long
+    R.csq_peek.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_long]
+    R.csq_advance.argtypes = [ctypes.c_void_p]
+    R.csq_size.restype = ctypes.c_long
+    R.csq_size.argtypes = [ctypes.c_void_p]
+    rng = np.random.default_rng(5)
+    P = ctypes.POINTER
+    for trial in range(16):
+        n_src, start = int(rng.integers(1, 30)), int(rng.integers(0, 4))
+        S = int(rng.integers(1, 300))
+        sources = rng.integers(start, start + n_src, S).astype(np.int32)
+        delays = (rng.integers(0, 40, S) * 1e-4).astype(np.float64)
+        rq, oq = R.csq_create(start, start + n_src), O.Queue(start, start + n_src)
+        dt = 1e-4
+        for run in range(4):
+            R.csq_prepare(rq, delays.ctypes.data_as(P(ctypes.c_double)), S, sources.ctypes.data_as(P(ctypes.c_int32)),
+                          S, dt)
+            oq.prepare(delays, sources, dt)
+            assert R.csq_size(rq) == oq.size(), (trial, run)
+            for step in range(int(rng.integers(3, 30))):
+                spk = np.sort(rng.choice(np.arange(start, start + n_src), min(n_src, int(rng.integers(0, 5))),
+                                         replace=False)).astype(np.int32)
+                R.csq_push(rq, spk.ctypes.data_as(P(ctypes.c_int32)), len(spk))
+                oq.push(spk)
+                buf = np.empty(100000, np.int32)
+                n = R.csq_peek(rq, buf.ctypes.data_as(P(ctypes.c_int32)), len(buf))
+                assert np.array_equal(buf[:n], oq.peek()), (trial, run, step)
+                R.csq_advance(rq)
+                oq.advance()
+            # factors old/new dt of at least 0.5 (collisions of old slots included); below 0.5 the
+            # reference can index past its vector (see hbr_queue_prepare)
+            dt = dt * float(rng.choice([0.1, 0.25, 0.5, 2.0 / 3.0, 1.0, 1.5, 2.0]))
+        R.csq_destroy(rq)
+
+
+def test_oracle_dt_changes_bitexact_vs_brian():
+    """The CUBA-with-delays model run in four pieces with dt changed between them
+    (0.1, 0.05, 0.1, 0.2 ms; spikes in flight at every change), against Brian's
+    cython runtime: spike indices and times, final state and lastspike."""
+    z, meta = O.load_golden("cuba_dt_change")
+    i, t, G = O.run_dt_changes(meta)
+    assert len(i) > 1000
+    assert np.array_equal(i, z["i"]) and np.array_equal(t, z["t"])
+    assert np.array_equal(G.state, z["final"])
+    assert np.array_equal(G.lastspike, z["lastspike"])
+
+
 def test_oracle_queue_dt_change_rebins():
     """spikequeue.h:58-74: queued spikes are re-binned when dt changes between runs."""
     q = O.Queue(0, 3)
@@ -354,6 +414,21 @@ def _oracle_kat(case):
 def test_oracle_transmission_kats(name):
     case = kats.CASES[name]
     assert np.array_equal(_oracle_kat(case), kats.expected_matrix(case))
+
+
+def test_oracle_subgroup_propagation_kat():
+    """test_subgroup.py:718-729: neurons 1, 3, 5 of G1 spike at step 0 (v = 1.1 > 1);
+    Synapses(G1[1:6], G2[10:]) connected i == j within the subgroups (absolute
+    indices 1..5 -> 10..14), on_pre v += 1: G2.v is 1 at 10, 12, 14 after one step."""
+    src = O.OracleGroup(O.HBR_SPIKEGEN, 10, np.zeros((1, 10)), [0.0],
+                        gen=(np.array([1, 3, 5], np.int32), np.zeros(3, np.int32)))
+    tgt = O.OracleGroup(O.HBR_PASSIVE, 20, np.zeros((1, 20)), [0.0])
+    path = O.OraclePathway(src, tgt, np.arange(1, 6), np.arange(10, 15), 0, w_scalar=1.0, delay_scalar=0.0,
+                           source_range=(1, 6))
+    O.run([src, tgt], [path], kats.DT, 1, mon_group=None)
+    expected = np.zeros(20)
+    expected[[10, 12, 14]] = 1
+    assert np.array_equal(tgt.state[0], expected)
 
 
 def test_burst_fold_restatement_equals_oracle():
