@@ -65,6 +65,7 @@ _SIGS = {
     "hb_poisson_function": ([_P(_vp)], _i),
     "hb_csr_count": ([_vp, _vp, _i64, _i32, _i32, _i32, _i32, _vp, _vp], _i),
     "hb_scan_u32": ([_vp, _i64, _vp], _i),
+    "hb_scan_u32_noalloc": ([_vp, _i64, _vp], _i),
     "hb_gen_fixed_indegree": ([_u64, _u64, _i32, _i32, _i32, _vp, _vp, _vp], _i),
     "hb_csr_from_indegree": ([_vp, _i32, _i32, _i32, _i32, _u64, _u64, _i32, _i32, _u64, _dbl, _dbl, _dbl, _vp, _vp,
                               _vp, _vp, _vp], _i),

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,8 @@ static int set_err(const std::string& m) {
         hipError_t e_ = (call);                                                             \
         if (e_ != hipSuccess) return set_err(std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+static int hb_scratch_for_current_device(uint32_t** out);
 
 extern "C" int hb_version(void) { return 1; }
 
@@ -74,6 +77,8 @@ extern "C" int hb_ctx_create(int device, hb_ctx** out) {
     HB_HIP(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
     HB_HIP(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
     HB_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    uint32_t* scr = nullptr;
+    if (hb_scratch_for_current_device(&scr)) return -1;
     *out = c.release();
     return 0;
 }
@@ -780,6 +785,55 @@ extern "C" int hb_scan_u32(uint32_t* d, int64_t n, void* stream) {
     hipLaunchKernelGGL(hb_k_scan_apply, dim3((unsigned)nb), dim3(256), 0, s, d, n, sums, d + n);
     HB_HIP(hipGetLastError());
     HB_HIP(hipFreeAsync(sums, s));
+    return 0;
+}
+
+// The allocation-free scan keeps its block sums in a file-scope device array (one
+// copy per GPU, held until the process ends).  Host code never passes `scratch`
+// itself to a kernel: its host-side name is not a device address.  The address of
+// each GPU's copy is looked up once per device, with that device current
+// (hb_ctx_create, or the entry point's first call on a device), and used instead.
+#define HB_SCAN_SCRATCH_SUMS 65536  // block sums: up to 65536 * 1024 elements per call
+#define HB_MAX_DEVICES 64
+static __device__ __align__(256) uint32_t scratch[HB_SCAN_SCRATCH_SUMS];
+static uint32_t* g_scratch_ptr[HB_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int hb_scratch_for_current_device(uint32_t** out) {
+    int dev = 0;
+    HB_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= HB_MAX_DEVICES) return set_err("hb_scan_u32_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_err("hb_scan_u32_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = (uint32_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int hb_scan_u32_noalloc(uint32_t* d, int64_t n, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nmax = (int64_t)HB_SCAN_SCRATCH_SUMS * HB_SCAN_BLOCK;
+    if (n > nmax)
+        return set_err("hb_scan_u32_noalloc: " + std::to_string(n) + " counts exceed the maximum of " +
+                       std::to_string(nmax) + " (use hb_scan_u32)");
+    uint32_t* sums = nullptr;
+    if (hb_scratch_for_current_device(&sums)) return -1;
+    if (n <= 0) {
+        hipLaunchKernelGGL(hb_k_fill_u32, dim3(1), dim3(64), 0, s, d, 0u, (size_t)1);
+        HB_HIP(hipGetLastError());
+        return 0;
+    }
+    // every block sum the later kernels read is written by hb_k_scan_reduce first,
+    // so the scratch needs no clearing between calls
+    int64_t nb = (n + HB_SCAN_BLOCK - 1) / HB_SCAN_BLOCK;
+    hipLaunchKernelGGL(hb_k_scan_reduce, dim3((unsigned)nb), dim3(256), 0, s, d, n, sums);
+    hipLaunchKernelGGL(hb_k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nb);
+    hipLaunchKernelGGL(hb_k_scan_apply, dim3((unsigned)nb), dim3(256), 0, s, d, n, sums, d + n);
+    HB_HIP(hipGetLastError());
     return 0;
 }
 

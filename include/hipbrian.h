@@ -146,6 +146,14 @@ int hb_csr_count(const int32_t* pre, const int32_t* delay_steps, int64_t n_syn, 
                  int32_t dmin, int32_t n_seg, uint32_t* seg_count, void* stream);
 /* In-place exclusive scan of n uint32 counts into n+1 offsets (last = total). */
 int hb_scan_u32(uint32_t* counts_to_offsets, int64_t n, void* stream);
+/* hb_scan_u32 without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight loops, harnesses that forbid hipMalloc).  Its block sums live in a
+ * fixed device array of the library (256 KiB per GPU, held until the process
+ * ends), so n is at most 65536 * 1024 = 67108864 counts; larger n is rejected
+ * with an error and nothing is launched.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (different streams or threads) race. */
+int hb_scan_u32_noalloc(uint32_t* counts_to_offsets, int64_t n, void* stream);
 
 /* Synthetic connectivity (deterministic, counter-based SplitMix64 streams):
  * fixed in-degree K per target, sources ascending per target. */
