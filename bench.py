@@ -55,10 +55,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n=20000, K=1000, warm=1000, steps=2000):
+def cpu_baseline(n=100000, K=1000, warm=1000, steps=1000):
     """The C restatement of the step (the oracle) on a bounded sample of the same
-    workload: the same model, in-degree, delays and weights at N = n neurons, one
-    thread and then every core of the host (OpenMP)."""
+    workload: the same model, in-degree, delays and weights at N = n neurons (a
+    tenth of the configuration), built and settled once, then the same steps timed
+    on one thread and on every core of the host (OpenMP), each from the settled
+    state with the settled spikes in flight."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import numpy as np
 
@@ -68,15 +70,18 @@ def cpu_baseline(n=20000, K=1000, warm=1000, steps=2000):
                 refractory=0.005, dt=DT, dmin=1, dmax=50, seed=1234, we_base=1.62e-3 * 80.0 / K,
                 wi_base=-9e-3 * 80.0 / K * 0.4)
     ncpu = O.cpu_threads()
+    G, paths = O.build_cuba_delay(meta)
+    for p in paths:
+        p.counts = np.zeros(len(p.pre), np.int32)
+    O.run([G], paths, meta["dt"], warm, mon_group=None, threads=ncpu)
+    settled = (G.state.copy(), G.lastspike.copy(), G.not_refractory.copy())
     out = {}
     for threads in (1, ncpu):
-        G, paths = O.build_cuba_delay(meta)
-        for p in paths:
-            p.counts = np.zeros(len(p.pre), np.int32)
-        O.run([G], paths, meta["dt"], warm, mon_group=None, threads=threads)
+        G.state[:], G.lastspike[:], G.not_refractory[:] = settled
+        qs = O.OracleQueues(len(paths))  # fresh queues: the first 50 steps miss the settle's spikes in flight
         before = sum(int(p.counts.sum()) for p in paths)
         t0 = time.perf_counter()
-        O.run([G], paths, meta["dt"], steps, start_step=warm, mon_group=None, threads=threads)
+        O.run([G], paths, meta["dt"], steps, start_step=warm, mon_group=None, threads=threads, queues=qs)
         wall = time.perf_counter() - t0
         events = sum(int(p.counts.sum()) for p in paths) - before
         out[threads] = (events / wall, events, wall, G.state.copy())
