@@ -46,6 +46,10 @@ KEY_TGT_SHIFT = 48
 MAX_RANK = 16
 MAX_SEG = 4095
 BIG_LISTED = 64  # targets per tile and step ordered cooperatively when their events exceed cap/2
+FOLD_LOADS_COUNT = 8  # fold: loads in flight per thread when counting the events beyond the registers
+FOLD_LOADS_PASS = 2   # fold: loads in flight per thread when a pass reads them again
+FOLD_RANK_UNROLL = 1  # fold: keys of a segment read per LDS round trip when ranking an event
+FOLD_APPLY_BATCH = 3  # fold: events of a target read together before they are applied in order
 GATHER_BLOCK = 1024
 DECL_MARK = "// ---- generated declarations ----"  # the per-kernel part of a source starts here
 
@@ -121,7 +125,7 @@ def _rename(lines, names):
     return [pat.sub(lambda m: names[m.group(1)], ln) for ln in lines]
 
 
-FOLD_EVENTS_PER_THREAD = 2    # fold: events a thread keeps in registers
+FOLD_EVENTS_PER_THREAD = 1    # fold: events a thread keeps in registers
 GATHER_EVENTS_PER_THREAD = 8  # gather: events a thread keeps in registers per window
 GATHER_STAGE_RUNS = True      # gather: sort a window's events by tile in LDS before writing them
 LDS_BYTES = 163840            # LDS of one CU
@@ -237,9 +241,20 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "    #pragma unroll",
         f"    for (int hb_u = 0; hb_u < {U}; hb_u++)",
         f"        if (hb_u * {T} + (int)threadIdx.x < hb_tot) atomicAdd(&s_fill[(int)(hb_fk[hb_u] >> {KEY_TGT_SHIFT})], 1);",
-        "    // events beyond the registers: counted here, read again by the passes",
-        f"    for (int g = {XW} + (int)threadIdx.x; g < hb_tot; g += {T})",
-        f"        atomicAdd(&s_fill[(int)(hb_rkey(hb_st[g]) >> {KEY_TGT_SHIFT})], 1);",
+        "    // events beyond the registers: counted here, read again by the passes; FOLD_LOADS_COUNT",
+        "    // loads in flight per thread (a load waited for before the next is issued leaves the",
+        "    // loop bound by the memory latency)",
+        f"    for (int g0 = {XW} + (int)threadIdx.x; g0 < hb_tot; g0 += {FOLD_LOADS_COUNT * T}) {{",
+        f"        unsigned int hb_kh[{FOLD_LOADS_COUNT}];",
+        "        #pragma unroll",
+        f"        for (int j = 0; j < {FOLD_LOADS_COUNT}; j++) {{",
+        f"            const int g = g0 + j * {T};",
+        "            hb_kh[j] = g < hb_tot ? (unsigned int)(hb_rkey(hb_st[g]) >> 32) : 0u;",
+        "        }",
+        "        #pragma unroll",
+        f"        for (int j = 0; j < {FOLD_LOADS_COUNT}; j++)",
+        f"            if (g0 + j * {T} < hb_tot) atomicAdd(&s_fill[(int)(hb_kh[j] >> {KEY_TGT_SHIFT - 32})], 1);",
+        "    }",
         "    __syncthreads();",
         "    // phase B: first event of every target (segments in (thread, q) order); passes",
         f"    int hb_c[{npt}], hb_b[{npt}], hb_myp[{npt}];",
@@ -301,6 +316,7 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
     sorted in global memory come last."""
     XW = T * U
     TN = T * npt
+    RU, AB = FOLD_RANK_UNROLL, FOLD_APPLY_BATCH
     k = [
         f"    const int hb_lim = hb_one ? hb_tot : {cap};  // LDS positions a pass may have written",
         "    for (int hk_pass = 0; hk_pass < hb_np; hk_pass++) {",
@@ -310,9 +326,16 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         f"            const int g = hb_u * {T} + (int)threadIdx.x;",
         "            if (g < hb_tot) HB_SCATTER(hb_fk[hb_u], hb_fv[hb_u], (unsigned int)g)",
         "        }",
-        f"        for (int g = {XW} + (int)threadIdx.x; g < hb_tot; g += {T}) {{",
-        "            const auto r_ = hb_st[g];",
-        f"            HB_SCATTER(hb_rkey(r_), hb_rval<{vt}>(r_), (unsigned int)g)",
+        f"        for (int g0 = {XW} + (int)threadIdx.x; g0 < hb_tot; g0 += {FOLD_LOADS_PASS * T}) {{",
+        f"            {rec_type(vt)} hb_rr[{FOLD_LOADS_PASS}];",
+        "            #pragma unroll",
+        f"            for (int j = 0; j < {FOLD_LOADS_PASS}; j++)",
+        f"                if (g0 + j * {T} < hb_tot) hb_rr[j] = hb_st[g0 + j * {T}];",
+        "            #pragma unroll",
+        f"            for (int j = 0; j < {FOLD_LOADS_PASS}; j++) {{",
+        f"                const int g = g0 + j * {T};",
+        f"                if (g < hb_tot) HB_SCATTER(hb_rkey(hb_rr[j]), hb_rval<{vt}>(hb_rr[j]), (unsigned int)g)",
+        "            }",
         "        }",
         "        __syncthreads();",
         f"        for (int x = threadIdx.x; x < hb_lim; x += {T}) {{",
@@ -321,24 +344,47 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         f"            t_ = t_ < {TN} ? t_ : {TN - 1};",
         "            const int bt_ = s_beg[t_], b_ = bt_ - hk_pb, c_ = s_fill[t_], e_ = b_ + c_ - 1;",
         "            if (hb_one || (bt_ >= 0 && (unsigned int)b_ < (unsigned int)hb_pw && x >= b_ && x <= e_)) {",
+        "                // smaller keys of the segment (FOLD_RANK_UNROLL loads per LDS round trip)",
         "                int r = 0;",
-        "                if (c_ > 1)",
-        "                    for (int j = b_; j <= e_; j++) r += s_key[j] < kx ? 1 : 0;",
+        "                if (c_ > 1) {",
+        "                    int j = b_;",
+    ]
+    if RU > 1:
+        k += [
+            f"                    for (; j + {RU - 1} <= e_; j += {RU}) {{",
+            "                        " + " ".join(f"const unsigned long long k{i}_ = s_key[j + {i}];" for i in range(RU)),
+            "                        r += " + " + ".join(f"(k{i}_ < kx ? 1 : 0)" for i in range(RU)) + ";",
+            "                    }",
+        ]
+    k += [
+        "                    for (; j <= e_; j++) r += s_key[j] < kx ? 1 : 0;",
+        "                }",
         "                s_x[b_ + r] = (unsigned short)(x - b_);",
         "            }",
         "        }",
         "        __syncthreads();",
-        "        // apply: the owner of each target applies its events in key order (the next",
-        "        // event's index is read while the current one is applied)",
+        "        // apply: the owner of each target applies its events in key order, FOLD_APPLY_BATCH",
+        "        // at a time (their order indices, then their keys and values read together)",
         "        #pragma unroll",
         f"        for (int q = 0; q < {npt}; q++) {{",
         "            if (!hb_bigt[q] && hb_myp[q] == hk_pass) {",
         "                const int o = hb_b[q] - hk_pb, c = hb_c[q];",
-        "                int a = c > 0 ? o + (int)s_x[o] : 0;",
-        "                for (int r = 0; r < c; r++) {",
-        "                    const unsigned long long hb_k = s_key[a];",
-        f"                    const {vt} hb_v = s_val[a];",
-        "                    a = r + 1 < c ? o + (int)s_x[o + r + 1] : a;",
+        "                int r = 0;",
+        f"                for (; r + {AB} <= c; r += {AB}) {{",
+        "                    " + " ".join(f"const int a{i}_ = o + (int)s_x[o + r + {i}];" for i in range(AB)),
+        "                    " + " ".join(f"const unsigned long long k{i}_ = s_key[a{i}_];" for i in range(AB)),
+        "                    " + " ".join(f"const {vt} v{i}_ = s_val[a{i}_];" for i in range(AB)),
+    ]
+    for i in range(AB):
+        k.append(f"                    {{ const unsigned long long hb_k = k{i}_; const {vt} hb_v = v{i}_;")
+        k += ["                        " + x for x in apply_lines]
+        k.append("                    }")
+    k += [
+        "                }",
+        "                for (; r < c; r++) {",
+        "                    const int a_ = o + (int)s_x[o + r];",
+        "                    const unsigned long long hb_k = s_key[a_];",
+        f"                    const {vt} hb_v = s_val[a_];",
     ]
     k += ["                    " + x for x in apply_lines]
     k += [

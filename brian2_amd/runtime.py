@@ -1086,10 +1086,14 @@ class Network:
             for s in g.incoming:
                 self._gather_units(s, _kernels.GATHER_BLOCK * ept)
                 self._gather_setup(s, ept)
-            # a quarter of the workgroup slots per pathway (measured on the 1M-neuron network:
-            # fewer starve a pathway of workgroups, more delay the tiles)
+            # 3/8 of the workgroup slots for the gathers, split over the pathways by their
+            # synapses (their shares of the events when the sources fire alike).  Measured on
+            # the 1M-neuron network (80 % of the synapses excitatory): more gather workgroups
+            # hold slots the tiles need, fewer leave the excitatory gather the last to finish
             wgs = 2 if waves else 1
-            blocks = cfg["gather_blocks"] or [max(1, self.device.n_cu * wgs // 4)] * len(g.incoming)
+            total = max(len(g.incoming), self.device.n_cu * wgs * 3 // 8)
+            nsyn = [max(1, int(s.csr["n_syn"])) for s in g.incoming]
+            blocks = cfg["gather_blocks"] or [max(1, round(total * n / sum(nsyn))) for n in nsyn]
             if len(blocks) != len(g.incoming):
                 raise ValueError(f"gather_blocks has {len(blocks)} entries for {len(g.incoming)} pathways into "
                                  f"{g.name}")
