@@ -296,10 +296,11 @@ def measured_traffic(kernel, sha1, workload):
         return None
     with open(tfile) as f:
         t = json.load(f)
-    e = t.get(kernel)
-    if not e or e.get("kernel_sha1") != sha1 or e.get("workload") != workload:
-        return None
-    return {k: e[k] for k in ("fetch_bytes", "write_bytes", "bytes", "dispatches", "kernel_sha1") if k in e}
+    entries = t.get(kernel) or []
+    for e in entries if isinstance(entries, list) else [entries]:
+        if e.get("kernel_sha1") == sha1 and e.get("workload") == workload:
+            return {k: e[k] for k in ("fetch_bytes", "write_bytes", "bytes", "dispatches", "kernel_sha1") if k in e}
+    return None
 
 
 def _rocprof_rows(d, suffix):
