@@ -945,12 +945,13 @@ class GatherKernel:
         lds = WN * 2 + NT * 8 + (NSR + 1) * 4 + PC * 10 + 1024
         if lds > LDS_BYTES:
             raise NotImplementedError("too many target tiles for the gather's LDS")
-        self.staged = GATHER_STAGE_RUNS and lds + WN * (8 + vb) <= LDS_BYTES
-        self.lds_bytes = lds + (WN * (8 + vb) if self.staged else 0)
+        self.staged = GATHER_STAGE_RUNS and lds + WN * (12 + vb) <= LDS_BYTES
+        self.lds_bytes = lds + (WN * (12 + vb) if self.staged else 0)
         # workgroups that share a CU: LDS, and 16 waves of at most 128 VGPRs
         self.wgs_per_cu = max(1, min(LDS_BYTES // self.lds_bytes, 1024 // GT))
         if self.staged:
-            k += [f"    __shared__ unsigned long long s_ok[{WN}];", f"    __shared__ {VT} s_ov[{WN}];"]
+            k += [f"    __shared__ unsigned long long s_ok[{WN}];", f"    __shared__ {VT} s_ov[{WN}];",
+                  f"    __shared__ unsigned int s_dst[{WN}];"]
         k += [
             f"    __shared__ __align__(16) int s_sh[{GT // 64 + 1}];",
             "    __shared__ int s_ovw;  // a run of this window overflows its region",
@@ -1112,25 +1113,29 @@ class GatherKernel:
         ]
         if self.staged:
             k += [
-                "            // the run sorted by tile in LDS (s_pid = tile of each sorted event), then written",
-                "            // out run by run, consecutive threads on consecutive positions",
+                "            // the run sorted by tile in LDS with each event's buffer position (s_dst), then",
+                "            // written out run by run, consecutive threads on consecutive positions.  s_hist[t]",
+                "            // becomes the start of tile t's run in the sorted window and s_base[t] the buffer",
+                "            // position of its run minus that start",
                 "            int carry = 0;",
                 f"            for (int t0 = 0; t0 < {NT}; t0 += {GT}) {{",
                 "                const int t = t0 + (int)threadIdx.x;",
                 f"                const int v = t < {NT} ? s_hist[t] : 0;",
                 "                int tot;",
                 "                const int x = hb_block_exscan(v, s_sh, &tot) + carry;",
-                f"                if (t < {NT}) s_hist[t] = x;",
+                f"                if (t < {NT}) {{ s_hist[t] = x; s_base[t] -= (unsigned int)x; }}",
                 "                carry += tot;",
                 "            }",
                 "            __syncthreads();",
                 "            #pragma unroll",
                 f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
                 "                if (hb_gt[hb_u] >= 0) {",
-                "                    const int q = atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
+                "                    const int t = hb_gt[hb_u];",
+                "                    const int q = atomicAdd(&s_hist[t], 1);",
                 "                    s_ok[q] = hb_gk[hb_u];",
                 "                    s_ov[q] = hb_gv[hb_u];",
-                "                    s_pid[q] = (unsigned short)hb_gt[hb_u];",
+                "                    const unsigned int r = s_base[t] + (unsigned int)q;",
+                "                    s_dst[q] = HB_FITS(t, (unsigned long long)r) ? r : 0xffffffffu;  // no position is 2^32 - 1",
             ]
             if self.count_synapses:
                 k.append("                    p.syn_count[(unsigned int)hb_gk[hb_u]] += 1;")
@@ -1139,14 +1144,10 @@ class GatherKernel:
                 "            }",
                 "            __syncthreads();",
                 f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
-                "            // s_hist[t] is now the end of tile t's run in the window, s_hist[t - 1] its start;",
                 "            // carry = the events placed (the window's events with a valid target)",
                 f"            for (int x = threadIdx.x; x < carry; x += {GT}) {{",
-                "                const int t = s_pid[x];",
-                "                const unsigned long long r = (unsigned long long)s_base[t] + (unsigned int)(x - (t > 0 ? s_hist[t - 1] : 0));",
-                "                if (HB_FITS(t, r)) {",
-                "                    hb_rset(hb_st[r], s_ok[x], s_ov[x]);",
-                "                }",
+                "                const unsigned int r = s_dst[x];",
+                "                if (r != 0xffffffffu) hb_rset(hb_st[r], s_ok[x], s_ov[x]);",
                 "            }",
                 "            __syncthreads();",
             ]
