@@ -930,7 +930,9 @@ class Network:
         # two index lists per tile (a target's events, then the same ordered by key)
         g.tile_cap = int(0 if not big else min(need_tile, (budget // 4) // (8 * g.n_tiles)))
         stg_budget = budget - 8 * g.n_tiles * g.tile_cap
-        limit = int(min(total, 2**32 - 1, stg_budget // (3 * (8 + vbytes))))
+        # positions are 32-bit: a region's end plus a gather window's offset into it (below
+        # 2^17 events) stays below 2^32 - 1, which marks an event that does not fit
+        limit = int(min(total, 2**32 - 2**17, stg_budget // (3 * (8 + vbytes))))
         region = indeg_t if limit >= total else (indeg_t * limit) // max(total, 1)
         off = np.zeros(g.n_tiles + 1, np.int64)
         off[1:] = np.cumsum(region)
