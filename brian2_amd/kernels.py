@@ -1031,16 +1031,24 @@ class GatherKernel:
             "        int E;",
             "        const int ex = hb_block_exscan((int)len, s_sh, &E);",
             f"        if ((int)threadIdx.x < {PC}) {{ s_a[threadIdx.x] = a; s_o[threadIdx.x] = ex; s_sg[threadIdx.x] = (unsigned short)seg; }}",
+            "        // each pair writes its index over its events of the window (the first window's here,",
+            "        // before the barrier the unit needs anyway)",
+            "        if ((int)threadIdx.x < nq) {",
+            "            const int e_ = ex + (int)len;",
+            f"            const int b_ = e_ < {WN} ? e_ : {WN};",
+            "            for (int x_ = ex; x_ < b_; x_++) s_pid[x_] = (unsigned short)threadIdx.x;",
+            "        }",
             "        __syncthreads();",
             f"        for (int w0 = 0; w0 < E; w0 += {WN}) {{",
-            "            // each pair writes its index over its events of the window",
-            "            if ((int)threadIdx.x < nq) {",
-            "                const int a_ = ex > w0 ? ex : w0;",
-            "                const int e_ = ex + (int)len;",
-            f"                const int b_ = e_ < w0 + {WN} ? e_ : w0 + {WN};",
-            "                for (int x_ = a_; x_ < b_; x_++) s_pid[x_ - w0] = (unsigned short)threadIdx.x;",
+            "            if (w0 > 0) {",
+            "                if ((int)threadIdx.x < nq) {",
+            "                    const int a_ = ex > w0 ? ex : w0;",
+            "                    const int e_ = ex + (int)len;",
+            f"                    const int b_ = e_ < w0 + {WN} ? e_ : w0 + {WN};",
+            "                    for (int x_ = a_; x_ < b_; x_++) s_pid[x_ - w0] = (unsigned short)threadIdx.x;",
+            "                }",
+            "                __syncthreads();",
             "            }",
-            "            __syncthreads();",
             f"            int hb_gt[{U}];",
             f"            unsigned long long hb_gk[{U}];",
             f"            {VT} hb_gv[{U}];",
@@ -1149,6 +1157,9 @@ class GatherKernel:
                 "                const unsigned int r = s_dst[x];",
                 "                if (r != 0xffffffffu) hb_rset(hb_st[r], s_ok[x], s_ov[x]);",
                 "            }",
+                "            // the counts of the next window (the write-out reads neither)",
+                f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
+                "            if (threadIdx.x == 0) s_ovw = 0;",
                 "            __syncthreads();",
             ]
         else:
@@ -1171,12 +1182,12 @@ class GatherKernel:
                 "                }",
                 "            }",
                 "            __syncthreads();",
+                f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
+                "            if (threadIdx.x == 0) s_ovw = 0;",
+                "            __syncthreads();",
             ]
         k += [
             "#undef HB_FITS",
-            f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
-            "            if (threadIdx.x == 0) s_ovw = 0;",
-            "            __syncthreads();",
             "        }",
             "        if (E == 0 && hb_more) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
             "        if (threadIdx.x == 0) hb_ev += (unsigned long long)E;",
