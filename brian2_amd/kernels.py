@@ -1102,39 +1102,57 @@ class GatherKernel:
             "            #pragma unroll",
             f"            for (int hb_u = 0; hb_u < {U}; hb_u++) if (hb_gt[hb_u] >= 0) atomicAdd(&s_hist[hb_gt[hb_u]], 1);",
             "            __syncthreads();",
-            "            // space in the regions of the tiles the window touches: one global atomic per tile;",
-            "            // s_base = buffer position of the window's run.  A run that does not fit is written",
-            "            // up to the region's end (so the region has no unwritten hole) and reported",
-            f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) {{",
-            "                const unsigned int c = (unsigned int)s_hist[t];",
-            "                if (c > 0u) {",
-            "                    const unsigned int lo = p.tile_off[t], hi = p.tile_off[t + 1];",
-            f"                    const unsigned int b = atomicAdd(p.tile_cnt + (size_t)par * {NT} + t, c);",
-            "                    if ((unsigned long long)b + c > hi - lo) { s_ovw = 1; atomicOr(p.status, 1); }",
-            "                    s_base[t] = b < hi - lo ? lo + b : hi;  // position in the buffer",
-            "                }",
-            "            }",
-            "            __syncthreads();",
+        ]
+        fits = [
             "            const bool ovw = s_ovw != 0;",
             "            // an event at buffer position r of tile t fits unless r passes the region's end",
             "#define HB_FITS(t, r) (!ovw || (r) < (unsigned long long)p.tile_off[(t) + 1])",
         ]
-        if self.staged:
+        if not self.staged:
             k += [
-                "            // the run sorted by tile in LDS with each event's buffer position (s_dst), then",
-                "            // written out run by run, consecutive threads on consecutive positions.  s_hist[t]",
-                "            // becomes the start of tile t's run in the sorted window and s_base[t] the buffer",
-                "            // position of its run minus that start",
+                "            // space in the regions of the tiles the window touches: one global atomic per tile;",
+                "            // s_base = buffer position of the window's run.  A run that does not fit is written",
+                "            // up to the region's end (so the region has no unwritten hole) and reported",
+                f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) {{",
+                "                const unsigned int c = (unsigned int)s_hist[t];",
+                "                if (c > 0u) {",
+                "                    const unsigned int lo = p.tile_off[t], hi = p.tile_off[t + 1];",
+                f"                    const unsigned int b = atomicAdd(p.tile_cnt + (size_t)par * {NT} + t, c);",
+                "                    if ((unsigned long long)b + c > hi - lo) { s_ovw = 1; atomicOr(p.status, 1); }",
+                "                    s_base[t] = b < hi - lo ? lo + b : hi;  // position in the buffer",
+                "                }",
+                "            }",
+                "            __syncthreads();",
+            ] + fits
+        else:
+            k += [
+                "            // space in the regions of the tiles the window touches (one global atomic per tile,",
+                "            // in flight during the block scan) and the window sorted by tile in LDS with each",
+                "            // event's buffer position (s_dst), then written out run by run, consecutive",
+                "            // threads on consecutive positions.  s_hist[t] becomes the start of tile t's run in",
+                "            // the sorted window and s_base[t] the buffer position of the run minus that start.",
+                "            // A run that does not fit is written up to the region's end (so the region has no",
+                "            // unwritten hole) and reported",
                 "            int carry = 0;",
                 f"            for (int t0 = 0; t0 < {NT}; t0 += {GT}) {{",
                 "                const int t = t0 + (int)threadIdx.x;",
-                f"                const int v = t < {NT} ? s_hist[t] : 0;",
+                f"                const unsigned int c = t < {NT} ? (unsigned int)s_hist[t] : 0u;",
+                "                unsigned int lo = 0u, hi = 0u, b = 0u;",
+                "                if (c > 0u) {",
+                "                    lo = p.tile_off[t]; hi = p.tile_off[t + 1];",
+                f"                    b = atomicAdd(p.tile_cnt + (size_t)par * {NT} + t, c);",
+                "                }",
                 "                int tot;",
-                "                const int x = hb_block_exscan(v, s_sh, &tot) + carry;",
-                f"                if (t < {NT}) {{ s_hist[t] = x; s_base[t] -= (unsigned int)x; }}",
+                "                const int x = hb_block_exscan((int)c, s_sh, &tot) + carry;",
+                f"                if (t < {NT}) s_hist[t] = x;",
+                "                if (c > 0u) {",
+                "                    if ((unsigned long long)b + c > hi - lo) { s_ovw = 1; atomicOr(p.status, 1); }",
+                "                    s_base[t] = (b < hi - lo ? lo + b : hi) - (unsigned int)x;",
+                "                }",
                 "                carry += tot;",
                 "            }",
                 "            __syncthreads();",
+            ] + fits + [
                 "            #pragma unroll",
                 f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
                 "                if (hb_gt[hb_u] >= 0) {",
@@ -1162,7 +1180,7 @@ class GatherKernel:
                 "            if (threadIdx.x == 0) s_ovw = 0;",
                 "            __syncthreads();",
             ]
-        else:
+        if not self.staged:
             k += [
                 f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
                 f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",

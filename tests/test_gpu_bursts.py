@@ -75,3 +75,23 @@ def test_burst_trace_bitexact_with_concurrent_gathers(case, shape):
     print(f"burst case with delays >= 1, launch shape {shape}", flush=True)
     got = device_trace(pre, post, w, d1, **shape)
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("shape", [
+    {"tile": 64, "bucket_cap": 128, "gather_pairs": 32},
+    {"tile": 1024, "fuse": False},
+], ids=["tile64-cap128-pairs32", "tile1024-separate"])
+def test_burst_trace_bitexact_unsorted_gather_windows(case, shape, monkeypatch):
+    """The gathers' other write-out, for windows whose tile sort does not fit the LDS
+    (many target tiles and large windows): each event takes its place in its tile's
+    run with an LDS atomic and is written from registers.  Forced here by switching
+    the sort off; delays 1-21 steps."""
+    from brian2_amd import kernels
+
+    monkeypatch.setattr(kernels, "GATHER_STAGE_RUNS", False)
+    pre, post, w, d, _ = case
+    d1 = d + 1
+    exp = expected_trace(pre, post, w, d1)
+    print(f"burst case with delays >= 1, unsorted gather windows, launch shape {shape}", flush=True)
+    got = device_trace(pre, post, w, d1, **shape)
+    assert np.array_equal(got, exp)
