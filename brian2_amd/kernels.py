@@ -391,7 +391,9 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         "                }",
         "            }",
         "        }",
-        "        __syncthreads();",
+        "        // the next pass overwrites what this one's owners read (after the last pass the",
+        "        // LDS is next written behind a barrier)",
+        "        if (hk_pass + 1 < hb_np) __syncthreads();",
         "    }",
         "    // targets sorted in global memory (more than cap/2 events): the workgroup orders each",
         "    // one's index list by counting, for every event, the smaller keys of the target (keys",

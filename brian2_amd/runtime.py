@@ -944,11 +944,14 @@ class Network:
 
     def _gather_units(self, s, window):
         """(delay, spike) pairs per gather unit: about one window of events per unit on
-        average (s.pairs), and the most units one step can have (s.units_max)."""
+        average (s.pairs), and the most units one step can have (s.units_max).  A unit's
+        events are a sum over its pairs' segments, so they vary little about the mean:
+        units of 95 % of a window rarely need a second window (measured on the 1M-neuron
+        network: 0.95 of a window 46-48 us/step against 49 for 0.85)."""
         nsrc = s.src_range[1] - s.src_range[0]
         pairs_max = nsrc * s.csr["n_seg"]
         avg = s.csr["n_syn"] / max(1, pairs_max)
-        pc = int(min(_kernels.GATHER_BLOCK, max(32, 0.85 * window / max(avg, 1e-9))))
+        pc = int(min(_kernels.GATHER_BLOCK, max(32, 0.95 * window / max(avg, 1e-9))))
         s.pairs = int(self.gather_pairs or pc)
         s.units_max = -(-pairs_max // s.pairs)
 
