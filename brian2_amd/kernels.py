@@ -235,7 +235,9 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "    #pragma unroll",
         f"    for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
         f"        const int g = hb_u * {T} + (int)threadIdx.x;",
-        f"        if (g < hb_tot) {{ const auto r_ = hb_st[g]; hb_fk[hb_u] = hb_rkey(r_); hb_fv[hb_u] = hb_rval<{vt}>(r_); }}",
+        "        // loads under a condition are each waited for inside their branch: the index is",
+        "        // clamped instead (hb_tot > 0 here), and only the use is conditional",
+        f"        const auto r_ = hb_st[g < hb_tot ? g : hb_tot - 1]; hb_fk[hb_u] = hb_rkey(r_); hb_fv[hb_u] = hb_rval<{vt}>(r_);",
         "    }",
         "    __syncthreads();",
         "    #pragma unroll",
@@ -249,7 +251,7 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "        #pragma unroll",
         f"        for (int j = 0; j < {FOLD_LOADS_COUNT}; j++) {{",
         f"            const int g = g0 + j * {T};",
-        "            hb_kh[j] = g < hb_tot ? (unsigned int)(hb_rkey(hb_st[g]) >> 32) : 0u;",
+        "            hb_kh[j] = (unsigned int)(hb_rkey(hb_st[g < hb_tot ? g : hb_tot - 1]) >> 32);  // clamped: see phase A",
         "        }",
         "        #pragma unroll",
         f"        for (int j = 0; j < {FOLD_LOADS_COUNT}; j++)",
@@ -330,7 +332,7 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         f"            {rec_type(vt)} hb_rr[{FOLD_LOADS_PASS}];",
         "            #pragma unroll",
         f"            for (int j = 0; j < {FOLD_LOADS_PASS}; j++)",
-        f"                if (g0 + j * {T} < hb_tot) hb_rr[j] = hb_st[g0 + j * {T}];",
+        f"                hb_rr[j] = hb_st[g0 + j * {T} < hb_tot ? g0 + j * {T} : hb_tot - 1];  // clamped: see phase A",
         "            #pragma unroll",
         f"            for (int j = 0; j < {FOLD_LOADS_PASS}; j++) {{",
         f"                const int g = g0 + j * {T};",
@@ -990,33 +992,36 @@ class GatherKernel:
             "    // from the ring), then its synapse segment; a unit's pairs are read while the",
             "    // previous unit is processed (the source at its start, the segment before its last",
             "    // window is written out)",
+            "    // (the loads are unconditional, at a valid index for a thread without a pair, so that",
+            "    // they stay in flight: a load under a condition is waited for inside its branch)",
             "#define HB_PAIR_SRC(c_, seg_, src_) { \\",
             f"        const int q_ = (c_) * {PC} + (int)threadIdx.x; \\",
-            "        src_ = -1; seg_ = 0; \\",
-            f"        if ((int)threadIdx.x < {PC} && q_ < P) {{ \\",
-            f"            int lo = 0, hi = {NSR}; \\",
-            "            while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (s_cum[m] <= q_) lo = m; else hi = m; } \\",
+            f"        const bool v_ = (int)threadIdx.x < {PC} && q_ < P; \\",
+            f"        int lo = 0, hi = {NSR}; \\",
+            "        if (v_) while (hi - lo > 1) { const int m = (lo + hi) >> 1; if (s_cum[m] <= q_) lo = m; else hi = m; } \\",
         ]
         if NR == 1 and RP == 1:
             k += [
-                "            seg_ = lo; \\",
-                f"            const int slot = (int)((step - {self.dmin} - lo) % p.ring_slots); \\",
-                "            src_ = p.ring_spk[(size_t)slot * p.ring_stride + (q_ - s_cum[lo])]; } }",
+                "        seg_ = lo; \\",
+                f"        const int slot = (int)((step - {self.dmin} - lo) % p.ring_slots); \\",
+                "        const int s_ = p.ring_spk[v_ ? (size_t)slot * p.ring_stride + (q_ - s_cum[lo]) : (size_t)0]; \\",
+                "        src_ = v_ ? s_ : -1; }",
             ]
         else:
             start = " : ".join(f"hb_j == {j} ? {b}" for j, b in enumerate(PB[:-1])) + (" : " if NR > 1 else "") + str(PB[-1])
             k += [
-                f"            seg_ = lo / {NR}; \\",
-                f"            const int hb_j = lo % {NR}; \\",
-                f"            const int slot = (int)((step - {self.dmin} - seg_) % p.ring_slots); \\",
-                f"            src_ = p.ring_spk[(size_t)slot * p.ring_stride + ({start}) + (q_ - s_cum[lo])]; (void)hb_j; }} }}",
+                f"        seg_ = lo / {NR}; \\",
+                f"        const int hb_j = lo % {NR}; \\",
+                f"        const int slot = (int)((step - {self.dmin} - seg_) % p.ring_slots); \\",
+                f"        const int s_ = p.ring_spk[v_ ? (size_t)slot * p.ring_stride + ({start}) + (q_ - s_cum[lo]) : (size_t)0]; \\",
+                "        src_ = v_ ? s_ : -1; (void)hb_j; }",
             ]
         k += [
             "#define HB_PAIR_SEG(seg_, src_, a_, len_) { \\",
-            "        a_ = 0u; len_ = 0u; \\",
-            "        if (src_ >= p.src0 && src_ < p.src1) { \\",
-            f"            const size_t row = (size_t)(src_ - p.src0) * {NS} + (seg_); \\",
-            "            a_ = p.seg_ptr[row]; len_ = p.seg_ptr[row + 1] - a_; } }",
+            "        const bool v_ = src_ >= p.src0 && src_ < p.src1; \\",
+            f"        const size_t row = v_ ? (size_t)(src_ - p.src0) * {NS} + (seg_) : (size_t)0; \\",
+            "        const unsigned int x0_ = p.seg_ptr[row], x1_ = p.seg_ptr[row + 1]; \\",
+            "        a_ = v_ ? x0_ : 0u; len_ = v_ ? x1_ - x0_ : 0u; }",
             "    int n_seg = 0, n_src = -1;",
             "    unsigned int n_a = 0u, n_len = 0u;",
             "    if ((int)blockIdx.x < n_units) {",
