@@ -47,7 +47,7 @@ MAX_RANK = 16
 MAX_SEG = 4095
 BIG_LISTED = 64  # targets per tile and step ordered cooperatively when their events exceed cap/2
 FOLD_LOADS_COUNT = 8  # fold: loads in flight per thread when counting the events beyond the registers
-FOLD_LOADS_PASS = 2   # fold: loads in flight per thread when a pass reads them again
+FOLD_LOADS_PASS = 4   # fold: loads in flight per thread when a pass reads them again
 FOLD_RANK_UNROLL = 1  # fold: keys of a segment read per LDS round trip when ranking an event
 FOLD_APPLY_BATCH = 3  # fold: events of a target read together before they are applied in order
 GATHER_BLOCK = 1024
