@@ -1178,10 +1178,20 @@ class GatherKernel:
                 "            __syncthreads();",
                 f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
                 "            // carry = the events placed (the window's events with a valid target)",
-                f"            for (int x = threadIdx.x; x < carry; x += {GT}) {{",
-                "                const unsigned int r = s_dst[x];",
-                "                if (r != 0xffffffffu) hb_rset(hb_st[r], s_ok[x], s_ov[x]);",
+                "            // (a window holds at most U events per thread: their positions, keys and values are",
+                "            // read from LDS together, then written)",
+                f"            unsigned int hb_wr[{U}];",
+                f"            unsigned long long hb_wk[{U}];",
+                f"            {VT} hb_wv[{U}];",
+                "            #pragma unroll",
+                f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
+                f"                const int x = hb_u * {GT} + (int)threadIdx.x;",
+                "                const int xc = x < carry ? x : 0;",
+                "                hb_wr[hb_u] = x < carry ? s_dst[xc] : 0xffffffffu; hb_wk[hb_u] = s_ok[xc]; hb_wv[hb_u] = s_ov[xc];",
                 "            }",
+                "            #pragma unroll",
+                f"            for (int hb_u = 0; hb_u < {U}; hb_u++)",
+                "                if (hb_wr[hb_u] != 0xffffffffu) hb_rset(hb_st[hb_wr[hb_u]], hb_wk[hb_u], hb_wv[hb_u]);",
                 "            // the counts of the next window (the write-out reads neither)",
                 f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
                 "            if (threadIdx.x == 0) s_ovw = 0;",
