@@ -47,7 +47,7 @@ MAX_RANK = 16
 MAX_SEG = 4095
 BIG_LISTED = 64  # targets per tile and step ordered cooperatively when their events exceed cap/2
 FOLD_LOADS_COUNT = 8  # fold: loads in flight per thread when counting the events beyond the registers
-FOLD_LOADS_PASS = 4   # fold: loads in flight per thread when a pass reads them again
+FOLD_LOADS_PASS = 3   # fold: loads in flight per thread when a pass reads them again
 FOLD_RANK_UNROLL = 1  # fold: keys of a segment read per LDS round trip when ranking an event
 FOLD_APPLY_BATCH = 3  # fold: events of a target read together before they are applied in order
 GATHER_BLOCK = 1024
@@ -300,6 +300,14 @@ def _fold_source(T, cap, vt, flush, npt=1):
         "                const int q_ = bt_ + atomicAdd(&s_fill[t_], 1) - hk_pb; \\",
         "                s_key[q_] = (kk); s_val[q_] = (vv); } \\",
         "        } else if (hk_pass == 0 && !hb_ovf) hb_go[~bt_ + atomicAdd(&s_fill[t_], 1)] = (pos); }",
+        "    // the same with the target's segment start already read (bt_)",
+        "#define HB_SCATTER_B(kk, vv, pos, bt_) { \\",
+        f"        const int t_ = (int)((kk) >> {KEY_TGT_SHIFT}); \\",
+        "        if ((bt_) >= 0) { \\",
+        "            if (hb_one || (unsigned int)((bt_) - hk_pb) < (unsigned int)hb_pw) { \\",
+        "                const int q_ = (bt_) + atomicAdd(&s_fill[t_], 1) - hk_pb; \\",
+        "                s_key[q_] = (kk); s_val[q_] = (vv); } \\",
+        "        } else if (hk_pass == 0 && !hb_ovf) hb_go[~(bt_) + atomicAdd(&s_fill[t_], 1)] = (pos); }",
     ]
     return pro, k, U, W
 
@@ -333,10 +341,14 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         "            #pragma unroll",
         f"            for (int j = 0; j < {FOLD_LOADS_PASS}; j++)",
         f"                hb_rr[j] = hb_st[g0 + j * {T} < hb_tot ? g0 + j * {T} : hb_tot - 1];  // clamped: see phase A",
+        "            // the records' segment starts first (one LDS round trip for the group)",
+        f"            int hb_bt[{FOLD_LOADS_PASS}];",
+        "            #pragma unroll",
+        f"            for (int j = 0; j < {FOLD_LOADS_PASS}; j++) hb_bt[j] = s_beg[(int)(hb_rkey(hb_rr[j]) >> {KEY_TGT_SHIFT})];",
         "            #pragma unroll",
         f"            for (int j = 0; j < {FOLD_LOADS_PASS}; j++) {{",
         f"                const int g = g0 + j * {T};",
-        f"                if (g < hb_tot) HB_SCATTER(hb_rkey(hb_rr[j]), hb_rval<{vt}>(hb_rr[j]), (unsigned int)g)",
+        f"                if (g < hb_tot) HB_SCATTER_B(hb_rkey(hb_rr[j]), hb_rval<{vt}>(hb_rr[j]), (unsigned int)g, hb_bt[j])",
         "            }",
         "        }",
         "        __syncthreads();",
@@ -454,6 +466,7 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         "        }",
         "    }",
         "#undef HB_SCATTER",
+        "#undef HB_SCATTER_B",
     ]
     return k
 
