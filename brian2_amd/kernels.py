@@ -1007,7 +1007,7 @@ class GatherKernel:
             "    // window is written out)",
             "    // (the loads are unconditional, at a valid index for a thread without a pair, so that",
             "    // they stay in flight: a load under a condition is waited for inside its branch)",
-            "#define HB_PAIR_SRC(c_, seg_, src_) { \\",
+            "#define HB_PAIR_SRC(c_, seg_, src_, ok_) { \\",
             f"        const int q_ = (c_) * {PC} + (int)threadIdx.x; \\",
             f"        const bool v_ = (int)threadIdx.x < {PC} && q_ < P; \\",
             f"        int lo = 0, hi = {NSR}; \\",
@@ -1017,8 +1017,7 @@ class GatherKernel:
             k += [
                 "        seg_ = lo; \\",
                 f"        const int slot = (int)((step - {self.dmin} - lo) % p.ring_slots); \\",
-                "        const int s_ = p.ring_spk[v_ ? (size_t)slot * p.ring_stride + (q_ - s_cum[lo]) : (size_t)0]; \\",
-                "        src_ = v_ ? s_ : -1; }",
+                "        src_ = p.ring_spk[v_ ? (size_t)slot * p.ring_stride + (q_ - s_cum[lo]) : (size_t)0]; ok_ = v_; }",
             ]
         else:
             start = " : ".join(f"hb_j == {j} ? {b}" for j, b in enumerate(PB[:-1])) + (" : " if NR > 1 else "") + str(PB[-1])
@@ -1026,20 +1025,22 @@ class GatherKernel:
                 f"        seg_ = lo / {NR}; \\",
                 f"        const int hb_j = lo % {NR}; \\",
                 f"        const int slot = (int)((step - {self.dmin} - seg_) % p.ring_slots); \\",
-                f"        const int s_ = p.ring_spk[v_ ? (size_t)slot * p.ring_stride + ({start}) + (q_ - s_cum[lo]) : (size_t)0]; \\",
-                "        src_ = v_ ? s_ : -1; (void)hb_j; }",
+                f"        src_ = p.ring_spk[v_ ? (size_t)slot * p.ring_stride + ({start}) + (q_ - s_cum[lo]) : (size_t)0]; \\",
+                "        ok_ = v_; (void)hb_j; }",
             ]
         k += [
-            "#define HB_PAIR_SEG(seg_, src_, a_, len_) { \\",
-            "        const bool v_ = src_ >= p.src0 && src_ < p.src1; \\",
+            "#define HB_PAIR_SEG(seg_, src_, ok_, a_, len_) { \\",
+            "        asm volatile(\"\" : \"+v\"(src_)); /* the source is read here, not earlier */ \\",
+            "        const bool v_ = ok_ && src_ >= p.src0 && src_ < p.src1; \\",
             f"        const size_t row = v_ ? (size_t)(src_ - p.src0) * {NS} + (seg_) : (size_t)0; \\",
             "        const unsigned int x0_ = p.seg_ptr[row], x1_ = p.seg_ptr[row + 1]; \\",
             "        a_ = v_ ? x0_ : 0u; len_ = v_ ? x1_ - x0_ : 0u; }",
             "    int n_seg = 0, n_src = -1;",
+            "    bool n_ok = false;",
             "    unsigned int n_a = 0u, n_len = 0u;",
             "    if ((int)blockIdx.x < n_units) {",
-            "        HB_PAIR_SRC((int)blockIdx.x, n_seg, n_src)",
-            "        HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
+            "        HB_PAIR_SRC((int)blockIdx.x, n_seg, n_src, n_ok)",
+            "        HB_PAIR_SEG(n_seg, n_src, n_ok, n_a, n_len)",
             "    }",
             "    for (int c = blockIdx.x; c < n_units; c += gridDim.x) {",
             f"        const int q0 = c * {PC};",
@@ -1047,7 +1048,7 @@ class GatherKernel:
             "        const unsigned int a = n_a, len = n_len;",
             "        const int seg = n_seg;",
             "        const bool hb_more = c + (int)gridDim.x < n_units;",
-            "        if (hb_more) HB_PAIR_SRC(c + (int)gridDim.x, n_seg, n_src)",
+            "        HB_PAIR_SRC(c + (int)gridDim.x, n_seg, n_src, n_ok)  // no pair for any thread without a next unit",
             "        int E;",
             "        const int ex = hb_block_exscan((int)len, s_sh, &E);",
             f"        if ((int)threadIdx.x < {PC}) {{ s_a[threadIdx.x] = a; s_o[threadIdx.x] = ex; s_sg[threadIdx.x] = (unsigned short)seg; }}",
@@ -1189,7 +1190,7 @@ class GatherKernel:
                 "                }",
                 "            }",
                 "            __syncthreads();",
-                f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
+                f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_ok, n_a, n_len)",
                 "            // carry = the events placed (the window's events with a valid target)",
                 "            // (a window holds at most U events per thread: their positions, keys and values are",
                 "            // read from LDS together, then written)",
@@ -1213,7 +1214,7 @@ class GatherKernel:
         if not self.staged:
             k += [
                 f"            for (int t = threadIdx.x; t < {NT}; t += {GT}) s_hist[t] = 0;",
-                f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
+                f"            if (hb_more && w0 + {WN} >= E) HB_PAIR_SEG(n_seg, n_src, n_ok, n_a, n_len)",
                 "            __syncthreads();",
                 "            #pragma unroll",
                 f"            for (int hb_u = 0; hb_u < {U}; hb_u++) {{",
@@ -1237,7 +1238,7 @@ class GatherKernel:
         k += [
             "#undef HB_FITS",
             "        }",
-            "        if (E == 0 && hb_more) HB_PAIR_SEG(n_seg, n_src, n_a, n_len)",
+            "        if (E == 0 && hb_more) HB_PAIR_SEG(n_seg, n_src, n_ok, n_a, n_len)",
             "        if (threadIdx.x == 0) hb_ev += (unsigned long long)E;",
             "    }",
             "#undef HB_PAIR_SRC",
