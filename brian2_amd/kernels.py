@@ -48,7 +48,7 @@ MAX_SEG = 4095
 BIG_LISTED = 64  # targets per tile and step ordered cooperatively when their events exceed cap/2
 FOLD_LOADS_COUNT = 8  # fold: loads in flight per thread when counting the events beyond the registers
 FOLD_LOADS_PASS = 3   # fold: loads in flight per thread when a pass reads them again
-FOLD_RANK_UNROLL = 1  # fold: keys of a segment read per LDS round trip when ranking an event
+FOLD_RANK_UNROLL = 3  # fold: keys of a segment read per LDS round trip when ranking an event
 FOLD_APPLY_BATCH = 3  # fold: events of a target read together before they are applied in order
 GATHER_BLOCK = 1024
 DECL_MARK = "// ---- generated declarations ----"  # the per-kernel part of a source starts here
@@ -358,7 +358,7 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         f"            t_ = t_ < {TN} ? t_ : {TN - 1};",
         "            const int bt_ = s_beg[t_], b_ = bt_ - hk_pb, c_ = s_fill[t_], e_ = b_ + c_ - 1;",
         "            if (hb_one || (bt_ >= 0 && (unsigned int)b_ < (unsigned int)hb_pw && x >= b_ && x <= e_)) {",
-        "                // smaller keys of the segment (FOLD_RANK_UNROLL loads per LDS round trip)",
+        f"                // smaller keys of the segment ({RU} per LDS round trip)",
         "                int r = 0;",
         "                if (c_ > 1) {",
         "                    int j = b_;",
