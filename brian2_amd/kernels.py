@@ -384,8 +384,12 @@ def _fold_passes(T, U, W, cap, vt, apply_lines, npt=1):
         "            if (!hb_bigt[q] && hb_myp[q] == hk_pass) {",
         "                const int o = hb_b[q] - hk_pb, c = hb_c[q];",
         "                int r = 0;",
+        "                // order slots one batch ahead (read at clamped positions: past the segment",
+        "                // they are not used)",
+        "                " + " ".join(f"int n{i}_ = (int)s_x[o + {i} < {cap - 1} ? o + {i} : {cap - 1}];" for i in range(AB)),
         f"                for (; r + {AB} <= c; r += {AB}) {{",
-        "                    " + " ".join(f"const int a{i}_ = o + (int)s_x[o + r + {i}];" for i in range(AB)),
+        "                    " + " ".join(f"const int a{i}_ = o + n{i}_;" for i in range(AB)),
+        "                    " + " ".join(f"n{i}_ = (int)s_x[o + r + {AB + i} < {cap - 1} ? o + r + {AB + i} : {cap - 1}];" for i in range(AB)),
         "                    " + " ".join(f"const unsigned long long k{i}_ = s_key[a{i}_];" for i in range(AB)),
         "                    " + " ".join(f"const {vt} v{i}_ = s_val[a{i}_];" for i in range(AB)),
     ]
